@@ -1,0 +1,168 @@
+"""Benchmark of the multi-fluid MHD patch update on one MI355X.
+
+Workload (default): the 3-D multi-fluid heliospheric box, 256^3 cells on one
+patch (+4 ghost cells), plasma + 3 neutral fluids (23 components), solar wind
+from an inner sphere r = 0.1 into the LISM (26 km/s, 3 uG at 45 degrees),
+charge exchange on, Roe 8-wave fluxes, minmod, RK2, 8-wave sources.
+Inputs live in HBM before the timed region; each step is one full RK2 patch
+update (both stages, sources, floors and the CFL reduction) with the state
+advanced in place; dt follows the CFL condition from the previous step.
+
+Prints one JSON line (metric cell-updates/s, roofline and CPU baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ms-flukss-multifluid_amd"))
+
+import numpy as np  # noqa: E402
+
+import mfmhd  # noqa: E402
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+
+
+def c3_params(n, prec):
+    dx = 1.0 / n
+    p = mfmhd.helio_params(lismN=0.06, lismV=2.6e6, lismT=6527.0, lismB=3e-6,
+                           lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0),
+                           lismUdir=(-1.0, 0.0, 0.0), sunN=7.0, sunV=4.5e7,
+                           sunXYZ=(0.5, 0.5, 0.5), R0=0.1, initR=0.3,
+                           netN=0.1, dx=dx, num_cells=(n, n, n))
+    p.nfluids = 4
+    p.time_approx = mfmhd.TIME_RK2
+    p.flux = mfmhd.FLUX_ROE8
+    p.precision = prec
+    return p, dx
+
+
+def cpu_baseline(n_sample, threads, budget_s=20.0):
+    """The CPU oracle (FP64 restatement of the reference's path) on a
+    bounded sample of the same workload: a n_sample^3 patch of the
+    heliospheric state around the sun, OpenMP over z planes."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    n = 256
+    p, dx = c3_params(n, mfmhd.PREC_FP64)
+    ng = 4
+    c0 = n // 2 - n_sample // 2
+    vlo = (c0, c0, c0)
+    vhi = (c0 + n_sample - 1,) * 3
+    lo = tuple(v - ng for v in vlo)
+    hi = tuple(v + ng for v in vhi)
+    shape = (23, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+    U = oracle.init_problem(p, shape, lo, lo, hi, dx)
+    dt0, _ = oracle.compute_dt(p, U, lo, vlo, vhi, dx)
+    dt = 0.5 * dt0
+    times = []
+    t_start = time.time()
+    while True:
+        t0 = time.perf_counter()
+        r = oracle.patch_update(p, U, lo, vlo, vhi, dt, dx, nthreads=threads)
+        times.append(time.perf_counter() - t0)
+        assert r["rc"] == 0
+        if time.time() - t_start > budget_s or len(times) >= 6:
+            break
+    med = float(np.median(times[1:] if len(times) > 2 else times))
+    cells = n_sample ** 3
+    return {"value": cells / med, "unit": "cell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": "oracle FP64 RK2 update of a %d^3 patch of the 256^3 "
+                      "4-fluid heliospheric state (+4 ghosts), median of %d "
+                      "steps, OpenMP %d threads" % (n_sample, len(times), threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--precision", choices=["fp32flux", "fp64"], default="fp32flux")
+    ap.add_argument("--cfl", type=float, default=0.8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(see DESIGN.md for how the patch table would shard)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    prec = mfmhd.PREC_FP32_FLUX if args.precision == "fp32flux" else mfmhd.PREC_FP64
+    n = args.n
+    p, dx = c3_params(n, prec)
+    ctx = mfmhd.Context(p)
+    nc = ctx.ncomp
+    ng = ctx.nghost
+    lo, hi = (-ng,) * 3, (n - 1 + ng,) * 3
+    m = n + 2 * ng
+    U = torch.empty((nc, m, m, m), dtype=torch.float64, device="cuda")
+    Uf = mfmhd.fab(U.data_ptr(), lo, hi, nc)
+    ctx.init_problem(Uf, mfmhd.Box.of(lo, hi), dx)
+    valid = mfmhd.Box.of((0, 0, 0), (n - 1,) * 3)
+    dtn, _ = ctx.compute_dt(Uf, valid, dx)
+    t = 0.0
+    for _ in range(args.warmup):
+        dt = args.cfl * dtn
+        dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
+        t += dt
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    w0 = time.perf_counter()
+    for _ in range(args.steps):
+        dt = args.cfl * dtn
+        dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
+        t += dt
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms = e0.elapsed_time(e1) / args.steps
+    cells = float(n) ** 3
+    value = cells / (ms * 1e-3)
+    alg_bytes = 5 * nc * 8  # per cell-update (RK2: read U, write U*, read U*, U, write U)
+    achieved = alg_bytes * value
+    out = {
+        "metric": "cell-updates/s per MF-MHD step",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 state, %s fluxes" % ("f32" if prec else "f64"),
+        "data": "synthetic (HELIOINIT state generated on the device)",
+        "config": {"workload": "3-D multi-fluid heliospheric box %d^3, plasma + 3 "
+                               "neutrals (23 comps), RK2, Roe8, minmod, 8-wave, "
+                               "charge exchange, single patch" % n,
+                   "cells": int(cells), "components": nc, "precision": args.precision},
+        "roofline": {"bound": "hbm", "kernel": "whole RK2 patch update",
+                     "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "alg_bytes_per_cell_update": alg_bytes},
+        "wall_s_timed": wall,
+    }
+    ctx.close()
+    del U
+    torch.cuda.empty_cache()
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cb = cpu_baseline(args.cpu_sample, threads)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = value / cb["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

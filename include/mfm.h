@@ -1,0 +1,226 @@
+/*
+ * mfm.h -- C ABI of the MI355X multi-fluid MHD patch integrator.
+ *
+ * The library advances one (or a table of) FArrayBox-shaped patches of
+ * conserved variables by one Godunov step, exactly as the reference patch
+ * integrator does:
+ *
+ *   PatchMHDAM::updateState   (MHDAM-3.2.4/src/PatchMHDAM.cpp:1194-1317,
+ *                              declared at src/PatchMHDAM.H:252-260)
+ *   PatchMHDMF::computeDt     (MHDAM-3.2.4/srcIdeal/PatchMHDMF.cpp:66-146)
+ *   PatchMHDAM::numGhostCells (MHDAM-3.2.4/src/PatchMHDAM.cpp:280-286)
+ *
+ * Data layout is the Chombo FArrayBox layout: component-major slabs, x
+ * fastest, index = (i-lo0) + n0*((j-lo1) + n1*((k-lo2) + n2*comp)).
+ * Boxes are cell-centred and inclusive (Chombo Box smallEnd/bigEnd).
+ * Every array pointer handed to an entry point that ends in _dev is a device
+ * (HBM) pointer; entry points that end in _host take host memory and stage it.
+ *
+ * The Fortran COMMON blocks the reference mutates (/eos/, /RiemSol/,
+ * /Dedner/, /chargeexcm/, /swlismcm/, /riemanncm/) are replaced by the
+ * immutable mfm_params block below, copied into the context at creation.
+ * Errors are returned as codes (the reference aborts through CH_assert /
+ * MayDay::Error); on error the state arrays are left unmodified.
+ */
+#ifndef MFM_H
+#define MFM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFM_ABI_VERSION 1
+
+/* ---- return codes ---------------------------------------------------- */
+enum {
+  MFM_OK = 0,
+  MFM_ERR_ARG = -1,        /* bad argument / unsupported option          */
+  MFM_ERR_SHAPE = -2,      /* box/ghost/ncomp inconsistent               */
+  MFM_ERR_DEVICE = -3,     /* HIP runtime failure                        */
+  MFM_ERR_NUMERIC = -4,    /* non-positive or non-finite dt (reference:  */
+                           /* CH_assert(dt > 0), src/PhysProblem.cpp:316) */
+  MFM_ERR_NOMEM = -5,
+  MFM_ERR_UNSUPPORTED = -6
+};
+
+/* ---- enumerations (values equal the reference's input-deck integers) -- */
+/* mhdam.flux (main/MHDAM.cpp:911-924, src/RiemannSolver.cpp:36-105)     */
+enum { MFM_FLUX_ROE8 = 1, MFM_FLUX_RUSANOV = 2, MFM_FLUX_ROE8_ROEAVG = 3,
+       MFM_FLUX_RUSANOV_ROEAVG = 4 };
+/* limiter1D::eLimiters (src/limiter1D.H:17-28)                           */
+enum { MFM_LIM_CD = 0, MFM_LIM_MINMOD = 1, MFM_LIM_SUPERBEE = 2,
+       MFM_LIM_VANALBADA = 3, MFM_LIM_MC = 4, MFM_LIM_HARMONIC = 5,
+       MFM_LIM_HYPERBEE = 6, MFM_LIM_KOREN = 7, MFM_LIM_VENKAT = 8,
+       MFM_LIM_VANLEER = 9, MFM_LIM_WENO3 = 10, MFM_LIM_WENO3YC = 11,
+       MFM_LIM_NONE = 12 /* constant (first-order) face values */ };
+/* mhdam.time_approximation (main/common.txt:118-122)                     */
+enum { MFM_TIME_FIRST_ORDER = 0, MFM_TIME_HANCOCK = 1, MFM_TIME_RK2 = 2 };
+/* mhdam.coord_sys (CoordinateSystemHandler::eCoordinateSystem)           */
+enum { MFM_COORD_CARTESIAN = 0, MFM_COORD_AXISYMMETRIC = 1 };
+/* physical problem supplying boundary conditions and problem sources     */
+enum { MFM_PROBLEM_NONE = 0,      /* all ghosts supplied by the caller    */
+       MFM_PROBLEM_RIEMANN = 1,   /* problems/RiemannProblem.cpp         */
+       MFM_PROBLEM_HELIO = 2 };   /* swlism/HeliosphericProblem.cpp      */
+/* arithmetic of reconstruction + Riemann fluxes                          */
+enum { MFM_PREC_FP64 = 0,         /* everything FP64 (parity build)       */
+       MFM_PREC_FP32_FLUX = 1 };  /* FP32 faces/fluxes, FP64 state, FP64 */
+                                  /* cons->prim, floors and accumulation  */
+
+/* ---- geometry -------------------------------------------------------- */
+typedef struct mfm_box { int lo[3]; int hi[3]; } mfm_box;
+
+/* An FArrayBox view.  data points to ncomp slabs over box. */
+typedef struct mfm_fab {
+  double* data;
+  mfm_box box;
+  int ncomp;
+} mfm_fab;
+
+/* ---- parameters (replace the reference's COMMON blocks) ---------------- */
+typedef struct mfm_params {
+  int abi_version;          /* must be MFM_ABI_VERSION                      */
+  int spacedim;             /* 2 or 3 (CH_SPACEDIM)                          */
+  /* equation system: EqSysMHDMF(iDedner, nFluids, 0)                       */
+  int nfluids;              /* 1..4 : plasma + (nfluids-1) neutral fluids  */
+  int dedner;               /* 1 = GLM psi component (divB_cleaning = 5)   */
+  /* numerics */
+  int flux;                 /* MFM_FLUX_*                                   */
+  int limiter[8];           /* limiter_RHO..limiter_BZ; all other slopes   */
+                            /* (neutrals, psi) use minmod                  */
+  int check_positivity;     /* mhdam.check_positivity                       */
+  int time_approx;          /* MFM_TIME_FIRST_ORDER or MFM_TIME_RK2         */
+  int divb_method;          /* mhdam.divB_method: 0 none, 1 faces, 2 CD     */
+  int eight_wave;           /* mhdam.8wave_use                              */
+  int coord_sys;            /* MFM_COORD_*                                  */
+  int precision;            /* MFM_PREC_*                                   */
+  double entropy_fix;       /* del2 (mhdam.entropy_fix_coeff, default 0.3)  */
+  double small_b;           /* smallB (1e-9, src/RiemannSolver.cpp:42)      */
+  /* /eos/ */
+  double gamma, smallr, smallp;
+  /* /Dedner/ initial values; the level driver updates them per step      */
+  double dedner_ch, dedner_cp;
+  /* problem domain (ProblemDomain): cell range and periodicity            */
+  int domain_lo[3], domain_hi[3], periodic[3];
+  int problem;              /* MFM_PROBLEM_*                               */
+  /* Riemann problem (SETRIEMANN): rho,u,v,w,p,bx,by,bz left / right       */
+  double rp_left[8], rp_right[8];
+  double rp_start_x;              /* initial discontinuity position (startX) */
+  /* Heliospheric problem (SETHELIOS output, nondimensional)               */
+  double sun_xc, sun_yc, sun_zc;  /* sun position (node-snapped)          */
+  double sun_r0;                  /* inner sphere radius R0               */
+  double sun_rho, sun_p, sun_v, sun_b, sun_tilt, sun_int_bc_radius;
+  double init_r;                  /* initial solar-wind sphere (initR)    */
+  double dir_br_n, omega, lism_v_dim;  /* lismV in cm/s for sunBeta       */
+  int    sun_b_monopolar;
+  double lism_ux, lism_uy, lism_uz, lism_p, lism_m, lism_t;
+  double lism_bx, lism_by, lism_bz;
+  double net_rho, net_p;
+  double tmlim, reg_safe_z_top, reg_safe_z_bot;
+  double r0_dt_ignore;            /* reserved: the dt exclusion radius is   */
+                                  /* R0 - 2 dx of the calling level, as in  */
+                                  /* HeliosphericProblem::define (:490)     */
+  /* /chargeexcm/ (SETCHARGEEX_PARS, swlism/ChargeExchange2F.ChF:4-21)     */
+  double ce_scale_vel;            /* lismV (cm/s)                         */
+  double ce_sigma_lindsay;        /* 1e-16 * scaleLen * lismN             */
+  int    ce_single_literals;      /* 1: Fortran default-real literals are */
+                                  /* REAL*4 (13.493, 0.531, ...)          */
+} mfm_params;
+
+/* sizeof(mfm_params), for FFI bindings to check their struct layout. */
+size_t mfm_params_sizeof(void);
+/* Fill p with the reference defaults (main/MHDAM.cpp:647-1046). */
+int mfm_params_default(mfm_params* p);
+/* Problem setters, restating SETRIEMANN (problems/RiemannF.ChF:4-63) and
+ * SETHELIOS + SETCHARGEEX_PARS (swlism/HeliosphericF.ChF:5-131,
+ * swlism/HeliosphericProblem.cpp:81-454).  Dimensional inputs as in the
+ * input decks; dx_level0 is the coarsest-level cell size. */
+int mfm_params_set_riemann(mfm_params* p, double gamma,
+                           const double left[8], const double right[8],
+                           double start_x);
+int mfm_params_set_orszag(mfm_params* p, double gamma);
+int mfm_params_set_helio(mfm_params* p, double gamma,
+                         double lismN, double lismV, double lismT, double lismB,
+                         const double lismBdir[3], const double lismUdir[3],
+                         const double sunXYZ[3], double R0, double sunN,
+                         double sunV, double sunT, double sunB, double sunTILT,
+                         double initR, double netN, double TMLIM,
+                         double regSafeDZ, double dx_level0,
+                         const int num_cells[3]);
+
+/* Number of state components for these parameters (EqSysMHDMF layout). */
+int mfm_num_components(const mfm_params* p);
+/* PatchMHDAM::numGhostCells: 4 (RK2), 3 (Hancock), 2 (first order). */
+int mfm_num_ghost_cells(const mfm_params* p);
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct mfm_ctx mfm_ctx;
+
+/* Replaces PatchMHDMF construction + define() + the COMMON-block setters.
+ * Binds to the current HIP device. */
+int mfm_ctx_create(const mfm_params* p, mfm_ctx** out);
+int mfm_ctx_destroy(mfm_ctx* ctx);
+/* EqSysMHDMF::setCorrectionPotential -> SETDEDNERCONST (src/DednerF.ChF:4-13) */
+int mfm_ctx_set_dedner(mfm_ctx* ctx, double ch, double cp);
+/* Last error message of this context (static storage, never NULL). */
+const char* mfm_ctx_last_error(const mfm_ctx* ctx);
+
+/* ---- one patch: PatchMHDAM::updateState ---------------------------------
+ * U      : device FAB, box = valid grown by >= numGhostCells, ghosts filled
+ *          (exchange / interpolation) as the reference requires; updated in
+ *          place on `valid`.  Ghost cells are left unchanged.
+ * F      : nullable; if given, spacedim device FABs on the face boxes of
+ *          `valid` (box hi[d] = valid.hi[d]+1) with ncomp comps receive the
+ *          time-averaged fluxes the reference returns for the flux registers.
+ * divB   : nullable device FAB on `valid` (1 comp) -- written when 8-wave
+ *          sources are on, as the reference does.
+ * dt     : time step; dx : cell size of this level; time : current time.
+ * new_dt : out, dx / max wave speed over `valid` (no CFL factor);
+ * min_cell: out, cell attaining it.
+ * stream : hipStream_t (NULL = default stream).  The call is asynchronous
+ *          except that it synchronises to return new_dt.                   */
+int mfm_patch_update_dev(mfm_ctx* ctx, mfm_fab U, mfm_box valid,
+                         double dt, double dx, double time,
+                         mfm_fab* F, mfm_fab* divB,
+                         double* new_dt, int min_cell[3], void* stream);
+/* Same contract with host FABs: stages U (and F, divB) through HBM.       */
+int mfm_patch_update_host(mfm_ctx* ctx, mfm_fab U, mfm_box valid,
+                          double dt, double dx, double time,
+                          mfm_fab* F, mfm_fab* divB,
+                          double* new_dt, int min_cell[3]);
+/* PatchMHDMF::computeDt over `box` (srcIdeal/PatchMHDMF.cpp:66-146). */
+int mfm_patch_compute_dt_dev(mfm_ctx* ctx, mfm_fab U, mfm_box box, double dx,
+                             double* dt, int min_cell[3], void* stream);
+
+/* ---- a level: the loop of AMRLevelIdealMHD::step over its boxes --------
+ * (srcIdeal/AMRLevelIdealMHD.cpp:1185-1325) as ONE launch sequence over a
+ * patch table.  Patches may have different sizes.                          */
+typedef struct mfm_patch {
+  mfm_fab U;          /* device FAB with ghosts                            */
+  mfm_box valid;      /* cells to advance                                  */
+  mfm_fab* F;         /* nullable: spacedim face FABs (device)            */
+  mfm_fab* divB;      /* nullable                                          */
+} mfm_patch;
+
+/* new_dt / min_cell: per-patch outputs (arrays of n / 3n), nullable.
+ * level_dt: min over patches, min_patch: its index.                        */
+int mfm_level_update_dev(mfm_ctx* ctx, const mfm_patch* table, int n,
+                         double dt, double dx, double time,
+                         double* new_dt, int* min_cell,
+                         double* level_dt, int* min_patch, void* stream);
+
+/* ---- initial data and ghost fill helpers (caller-side utilities) ------- */
+/* HELIOINIT (swlism/HeliosphericF.ChF:1150-1361) / ORSZAGINIT
+ * (problems/OrszagF.ChF:27-79) / RIEMANNINIT on a device FAB over `box`. */
+int mfm_init_problem_dev(mfm_ctx* ctx, mfm_fab U, mfm_box box, double dx,
+                         void* stream);
+/* Periodic ghost fill of a single-box periodic level (the part of
+ * LevelData::exchange that a one-box level needs).                         */
+int mfm_fill_periodic_dev(mfm_ctx* ctx, mfm_fab U, mfm_box valid, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFM_H */

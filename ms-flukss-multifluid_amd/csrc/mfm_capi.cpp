@@ -1,0 +1,819 @@
+// mfm_capi.cpp -- the C ABI (include/mfm.h): parameters, contexts, scratch
+// management and the launch sequence of PatchMHDAM::updateState over one
+// patch or a level's patch table.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/mfm.h"
+#include "mfm_device.hpp"
+#include "mfm_kernels.hpp"
+
+using namespace mfm;
+
+namespace {
+
+constexpr int kBlk = 256;
+const double eos_mp = 1.67262171e-24, eos_k = 1.3806505e-16;
+
+struct HBox {
+  int lo[3], hi[3];
+  long n(int d) const { return (long)hi[d] - lo[d] + 1; }
+  long size() const {
+    for (int d = 0; d < 3; ++d)
+      if (hi[d] < lo[d]) return 0;
+    return n(0) * n(1) * n(2);
+  }
+};
+HBox hb(const mfm_box& b) {
+  HBox r;
+  for (int d = 0; d < 3; ++d) { r.lo[d] = b.lo[d]; r.hi[d] = b.hi[d]; }
+  return r;
+}
+HBox grow(HBox b, int g, int dims) {
+  for (int d = 0; d < dims; ++d) { b.lo[d] -= g; b.hi[d] += g; }
+  return b;
+}
+bool contains(const HBox& a, const HBox& b) {
+  for (int d = 0; d < 3; ++d)
+    if (b.lo[d] < a.lo[d] || b.hi[d] > a.hi[d]) return false;
+  return true;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace
+
+struct mfm_ctx {
+  mfm_params p;
+  Consts C;
+  int device = 0;
+  std::string err = "";
+  // device arena for scratch + tables
+  char* arena = nullptr;
+  size_t arena_bytes = 0;
+  // pinned host staging for tables and results
+  char* hbuf = nullptr;
+  size_t hbuf_bytes = 0;
+  size_t fsize() const { return p.precision == MFM_PREC_FP32_FLUX ? 4 : 8; }
+};
+
+namespace {
+
+int fail(mfm_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  return code;
+}
+
+int hipfail(mfm_ctx* c, hipError_t e, const char* what) {
+  return fail(c, MFM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int ensure_arena(mfm_ctx* c, size_t bytes) {
+  if (bytes <= c->arena_bytes) return MFM_OK;
+  if (c->arena) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hipfail(c, e, "hipDeviceSynchronize");
+    (void)hipFree(c->arena);
+    c->arena = nullptr;
+    c->arena_bytes = 0;
+  }
+  size_t want = bytes + bytes / 8;
+  hipError_t e = hipMalloc((void**)&c->arena, want);
+  if (e != hipSuccess) {
+    c->arena = nullptr;
+    return fail(c, MFM_ERR_NOMEM, "hipMalloc of scratch failed");
+  }
+  c->arena_bytes = want;
+  return MFM_OK;
+}
+
+int ensure_hbuf(mfm_ctx* c, size_t bytes) {
+  if (bytes <= c->hbuf_bytes) return MFM_OK;
+  if (c->hbuf) {
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hipfail(c, e, "hipDeviceSynchronize");
+    (void)hipHostFree(c->hbuf);
+    c->hbuf = nullptr;
+    c->hbuf_bytes = 0;
+  }
+  size_t want = bytes + bytes / 4 + 4096;
+  hipError_t e = hipHostMalloc((void**)&c->hbuf, want, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    c->hbuf = nullptr;
+    return fail(c, MFM_ERR_NOMEM, "hipHostMalloc of staging failed");
+  }
+  c->hbuf_bytes = want;
+  return MFM_OK;
+}
+
+// Box &= ProblemDomain (periodic directions are not clipped)
+HBox clip(const Consts& C, HBox b) {
+  for (int d = 0; d < C.dims; ++d)
+    if (C.phys[d]) {
+      b.lo[d] = std::max(b.lo[d], C.dom_lo[d]);
+      b.hi[d] = std::min(b.hi[d], C.dom_hi[d]);
+    }
+  return b;
+}
+
+struct RegionBuilder {
+  std::vector<Region> v;
+  int nblk = 0;
+  void add(int patch, const HBox& b) {
+    long cells = b.size();
+    if (cells <= 0) return;
+    Region r;
+    r.patch = patch;
+    for (int d = 0; d < 3; ++d) { r.lo[d] = b.lo[d]; r.n[d] = (int)b.n(d); }
+    r.blk0 = nblk;
+    nblk += (int)((cells + kBlk - 1) / kBlk);
+    v.push_back(r);
+  }
+};
+
+int build_consts(const mfm_params* p, Consts& C, std::string& why) {
+  if (!p || p->abi_version != MFM_ABI_VERSION) { why = "abi_version mismatch"; return MFM_ERR_ARG; }
+  if (p->spacedim != 2 && p->spacedim != 3) { why = "spacedim must be 2 or 3"; return MFM_ERR_ARG; }
+  if (p->nfluids < 1 || p->nfluids > 4) { why = "nfluids must be 1..4"; return MFM_ERR_UNSUPPORTED; }
+  if (p->dedner && p->nfluids != 1) { why = "Dedner cleaning is single-fluid"; return MFM_ERR_UNSUPPORTED; }
+  if (p->flux < 1 || p->flux > 4) { why = "flux must be 1..4"; return MFM_ERR_ARG; }
+  if (p->dedner && !(p->flux == 1 || p->flux == 3)) {
+    why = "divB_cleaning = 5 must be used with flux = 1 or 3";  // main/MHDAM.cpp:1042-1046
+    return MFM_ERR_ARG;
+  }
+  if (p->dedner && p->eight_wave) { why = "divB_cleaning = 5 can not be used with 8wave_use = 1"; return MFM_ERR_ARG; }
+  if (p->time_approx != MFM_TIME_RK2 && p->time_approx != MFM_TIME_FIRST_ORDER) {
+    why = "time_approximation must be 0 (first order) or 2 (RK2)";
+    return MFM_ERR_UNSUPPORTED;
+  }
+  if (p->coord_sys != MFM_COORD_CARTESIAN) { why = "only Cartesian coordinates"; return MFM_ERR_UNSUPPORTED; }
+  if (p->precision != MFM_PREC_FP64 && p->precision != MFM_PREC_FP32_FLUX) { why = "bad precision"; return MFM_ERR_ARG; }
+  if (p->divb_method < 0 || p->divb_method > 2) { why = "bad divB_method"; return MFM_ERR_ARG; }
+  for (int v = 0; v < 8; ++v)
+    if (p->limiter[v] < 0 || p->limiter[v] > MFM_LIM_NONE) { why = "bad limiter"; return MFM_ERR_ARG; }
+  if (!(p->gamma > 1.0)) { why = "gamma must exceed 1"; return MFM_ERR_ARG; }
+  if (p->problem == MFM_PROBLEM_RIEMANN) {
+    if (p->nfluids != 1) { why = "Riemann problem is single-fluid here"; return MFM_ERR_UNSUPPORTED; }
+    for (int d = 1; d < p->spacedim; ++d)
+      if (!p->periodic[d]) { why = "Riemann problem: transverse directions must be periodic"; return MFM_ERR_UNSUPPORTED; }
+  } else if (p->problem == MFM_PROBLEM_HELIO) {
+    if (p->spacedim != 3) { why = "heliospheric problem is 3-D"; return MFM_ERR_UNSUPPORTED; }
+  } else if (p->problem != MFM_PROBLEM_NONE) {
+    why = "unknown problem"; return MFM_ERR_ARG;
+  }
+  std::memset(&C, 0, sizeof(C));
+  C.dims = p->spacedim;
+  C.nf = p->nfluids;
+  C.dedner = p->dedner ? 1 : 0;
+  C.nc = 8 + C.dedner + 5 * (C.nf - 1);
+  C.iPsi = C.dedner ? 8 : -1;
+  C.iRhoN = 8 + C.dedner;
+  C.flux_kind = p->dedner ? 2 : ((p->flux == 2 || p->flux == 4) ? 1 : 0);
+  C.iAveraging = (p->flux == 3 || p->flux == 4) ? 1 : 0;
+  C.eight_wave = p->eight_wave ? 1 : 0;
+  C.divb_method = p->divb_method;
+  C.problem = p->problem;
+  for (int c = 0; c < kMaxComp; ++c) C.lim[c] = (int8_t)(c < 8 ? p->limiter[c] : MFM_LIM_MINMOD);
+  C.gamma = p->gamma;
+  C.hgamma = p->gamma - 1.0;
+  C.invhgamma = 1.0 / C.hgamma;
+  C.smallr = p->smallr;
+  C.smallp = p->smallp;
+  C.del2 = p->entropy_fix;
+  C.smallB = p->small_b;
+  C.ch = p->dedner_ch;
+  C.cp = p->dedner_cp;
+  for (int d = 0; d < 3; ++d) {
+    C.dom_lo[d] = p->domain_lo[d];
+    C.dom_hi[d] = p->domain_hi[d];
+    C.phys[d] = (d < C.dims && p->problem != MFM_PROBLEM_NONE && !p->periodic[d]) ? 1 : 0;
+  }
+  for (int c = 0; c < 8; ++c) { C.rpL[c] = p->rp_left[c]; C.rpR[c] = p->rp_right[c]; }
+  C.rp_start_x = p->rp_start_x;
+  C.sun_xc = p->sun_xc; C.sun_yc = p->sun_yc; C.sun_zc = p->sun_zc;
+  C.sun_r0 = p->sun_r0; C.sun_rho = p->sun_rho; C.sun_p = p->sun_p;
+  C.sun_v = p->sun_v; C.sun_b = p->sun_b; C.sun_tilt = p->sun_tilt;
+  C.sun_int_bc_radius = p->sun_int_bc_radius; C.dir_br_n = p->dir_br_n;
+  C.omega = p->omega; C.lism_v_dim = p->lism_v_dim; C.init_r = p->init_r;
+  C.sun_b_monopolar = p->sun_b_monopolar;
+  C.lism_ux = p->lism_ux; C.lism_uy = p->lism_uy; C.lism_uz = p->lism_uz;
+  C.lism_p = p->lism_p; C.lism_m = p->lism_m; C.lism_t = p->lism_t;
+  C.lism_bx = p->lism_bx; C.lism_by = p->lism_by; C.lism_bz = p->lism_bz;
+  C.net_rho = p->net_rho; C.net_p = p->net_p; C.tmlim = p->tmlim;
+  C.reg_safe_z_top = p->reg_safe_z_top; C.reg_safe_z_bot = p->reg_safe_z_bot;
+  C.r0_dt_ignore = -1.0;
+  C.ce_scale_vel = p->ce_scale_vel;
+  C.ce_sigma = p->ce_sigma_lindsay;
+  // Fortran default-real literals of the cross section (see include/mfm.h)
+  C.c13493 = p->ce_single_literals ? (double)13.493f : 13.493;
+  C.c0531 = p->ce_single_literals ? (double)0.531f : 0.531;
+  C.ce_on = (p->problem == MFM_PROBLEM_HELIO && p->nfluids >= 2) ? 1 : 0;
+  return MFM_OK;
+}
+
+struct PatchIn {
+  mfm_fab U;
+  mfm_box valid;
+  mfm_fab* F;
+  mfm_fab* divB;
+};
+
+// Geometry of one patch's scratch.
+struct PatchGeo {
+  HBox ub, vb1, vb2, fb[3];
+  size_t off_up, off_w, off_f[3], off_bn[3];
+};
+
+int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
+               double time, double* new_dt, int* min_cell, double* level_dt,
+               int* min_patch, hipStream_t s) {
+  const Consts& C0 = c->C;
+  const mfm_params& p = c->p;
+  const int dims = C0.dims;
+  const int ng = p.time_approx == MFM_TIME_RK2 ? 4 : 2;
+  const bool rk2 = p.time_approx == MFM_TIME_RK2;
+  if (n <= 0) return fail(c, MFM_ERR_ARG, "empty patch table");
+  if (!(dt > 0.0) || !(dx > 0.0)) return fail(c, MFM_ERR_ARG, "dt and dx must be positive");
+  std::vector<PatchGeo> G(n);
+  size_t off = 0;
+  const size_t fs = c->fsize();
+  for (int q = 0; q < n; ++q) {
+    const PatchIn& P = in[q];
+    if (!P.U.data) return fail(c, MFM_ERR_ARG, "U.data is NULL");
+    if (P.U.ncomp != C0.nc) return fail(c, MFM_ERR_SHAPE, "U.ncomp does not match the equation system");
+    HBox ub = hb(P.U.box), v = hb(P.valid);
+    if (v.size() <= 0) return fail(c, MFM_ERR_SHAPE, "empty valid box");
+    for (int d = dims; d < 3; ++d)
+      if (ub.lo[d] != ub.hi[d] || v.lo[d] != v.hi[d] || ub.lo[d] != v.lo[d])
+        return fail(c, MFM_ERR_SHAPE, "unused dimensions must be a single cell");
+    if (!contains(ub, grow(v, ng, dims)))
+      return fail(c, MFM_ERR_SHAPE, "U box must contain the valid box grown by numGhostCells");
+    for (int d = 0; d < dims; ++d)
+      if (C0.phys[d] && (v.lo[d] < C0.dom_lo[d] || v.hi[d] > C0.dom_hi[d]))
+        return fail(c, MFM_ERR_SHAPE, "valid box extends outside the problem domain");
+    PatchGeo& g = G[q];
+    g.ub = ub;
+    g.vb2 = clip(C0, v);
+    g.vb1 = rk2 ? clip(C0, grow(v, 2, dims)) : g.vb2;
+    const long ucells = ub.size();
+    g.off_up = off; off += align256((size_t)ucells * C0.nc * 8);
+    g.off_w = off; off += align256((size_t)ucells * C0.nc * fs);
+    for (int d = 0; d < dims; ++d) {
+      g.fb[d] = g.vb1;
+      g.fb[d].hi[d] += 1;
+      g.off_f[d] = off; off += align256((size_t)g.fb[d].size() * C0.nc * fs);
+      g.off_bn[d] = off; off += align256((size_t)g.fb[d].size() * fs);
+    }
+    if (P.F) {
+      for (int d = 0; d < dims; ++d) {
+        HBox fo = hb(P.F[d].box), want = v;
+        want.hi[d] += 1;
+        if (!P.F[d].data || P.F[d].ncomp != C0.nc || !contains(fo, want))
+          return fail(c, MFM_ERR_SHAPE, "F[d] must cover the faces of the valid box with ncomp components");
+      }
+    }
+    if (P.divB && (!P.divB->data || P.divB->ncomp < 1))
+      return fail(c, MFM_ERR_SHAPE, "divB must have one component");
+  }
+  // region lists
+  RegionBuilder st1_prim, st1_flux[3], st1_upd, st2_prim, st2_flux[3], st2_upd;
+  for (int q = 0; q < n; ++q) {
+    const PatchGeo& g = G[q];
+    if (rk2) {
+      st1_prim.add(q, grow(g.vb1, 2, dims));
+      for (int d = 0; d < dims; ++d) st1_flux[d].add(q, g.fb[d]);
+      st1_upd.add(q, g.vb1);
+    }
+    st2_prim.add(q, grow(g.vb2, 2, dims));
+    for (int d = 0; d < dims; ++d) {
+      HBox f = g.vb2;
+      f.hi[d] += 1;
+      st2_flux[d].add(q, f);
+    }
+    st2_upd.add(q, g.vb2);
+  }
+  RegionBuilder* lists[10] = {&st1_prim, &st1_flux[0], &st1_flux[1], &st1_flux[2], &st1_upd,
+                              &st2_prim, &st2_flux[0], &st2_flux[1], &st2_flux[2], &st2_upd};
+  // table layout inside the arena (after the scratch)
+  size_t off_patch = off; off += align256(sizeof(DevPatch) * n);
+  size_t off_reg[10];
+  size_t table_bytes = align256(sizeof(DevPatch) * n);
+  for (int l = 0; l < 10; ++l) {
+    off_reg[l] = off;
+    size_t b = align256(sizeof(Region) * std::max<size_t>(1, lists[l]->v.size()));
+    off += b;
+    table_bytes += b;
+  }
+  const int nbu = std::max(1, st2_upd.nblk);
+  size_t off_dtp = off; off += align256(sizeof(double) * nbu);
+  size_t off_idp = off; off += align256(sizeof(long) * nbu);
+  size_t off_dto = off; off += align256(sizeof(double) * n);
+  size_t off_ido = off; off += align256(sizeof(long) * n);
+  int rc = ensure_arena(c, off);
+  if (rc) return rc;
+  rc = ensure_hbuf(c, table_bytes + align256(sizeof(double) * n) + align256(sizeof(long) * n) + 512);
+  if (rc) return rc;
+  char* A = c->arena;
+
+  // host staging of the tables
+  char* h = c->hbuf;
+  DevPatch* hp = reinterpret_cast<DevPatch*>(h);
+  const double sunR = p.sun_r0;
+  for (int q = 0; q < n; ++q) {
+    const PatchIn& P = in[q];
+    const PatchGeo& g = G[q];
+    DevPatch& D = hp[q];
+    std::memset(&D, 0, sizeof(D));
+    D.U = P.U.data;
+    D.Up = reinterpret_cast<double*>(A + g.off_up);
+    D.W = A + g.off_w;
+    for (int d = 0; d < 3; ++d) {
+      D.ulo[d] = g.ub.lo[d];
+      D.un[d] = (int)g.ub.n(d);
+    }
+    D.ucs = g.ub.size();
+    for (int d = 0; d < dims; ++d) {
+      D.F[d] = A + g.off_f[d];
+      D.Bn[d] = A + g.off_bn[d];
+      for (int e = 0; e < 3; ++e) { D.flo[d][e] = g.fb[d].lo[e]; D.fn[d][e] = (int)g.fb[d].n(e); }
+      D.fcs[d] = g.fb[d].size();
+      if (P.F) {
+        HBox fo = hb(P.F[d].box);
+        D.Fo[d] = P.F[d].data;
+        for (int e = 0; e < 3; ++e) { D.folo[d][e] = fo.lo[e]; D.fon[d][e] = (int)fo.n(e); }
+        D.focs[d] = fo.size();
+      }
+    }
+    if (P.divB) {
+      HBox db = hb(P.divB->box);
+      D.divB = P.divB->data;
+      for (int e = 0; e < 3; ++e) { D.dlo[e] = db.lo[e]; D.dn[e] = (int)db.n(e); }
+    }
+    // HELIOREINIT_DEFAULT's early exit on the FAB bounds
+    D.reinit = 0;
+    if (p.problem == MFM_PROBLEM_HELIO) {
+      double xl = g.ub.lo[0] * dx - p.sun_xc, yl = g.ub.lo[1] * dx - p.sun_yc, zl = g.ub.lo[2] * dx - p.sun_zc;
+      double xh = (g.ub.hi[0] + 1) * dx - p.sun_xc, yh = (g.ub.hi[1] + 1) * dx - p.sun_yc,
+             zh = (g.ub.hi[2] + 1) * dx - p.sun_zc;
+      bool out = (xl > sunR) || (yl > sunR) || (dims == 3 && zl > sunR) || (xh < -sunR) ||
+                 (yh < -sunR) || (dims == 3 && zh < -sunR);
+      D.reinit = out ? 0 : 1;
+    }
+  }
+  size_t hoff = align256(sizeof(DevPatch) * n);
+  for (int l = 0; l < 10; ++l) {
+    std::memcpy(h + hoff, lists[l]->v.data(), sizeof(Region) * lists[l]->v.size());
+    hoff += align256(sizeof(Region) * std::max<size_t>(1, lists[l]->v.size()));
+  }
+  hipError_t e = hipMemcpyAsync(A + off_patch, h, table_bytes, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hipfail(c, e, "hipMemcpyAsync(tables)");
+
+  auto rl = [&](int l) {
+    RegionList r;
+    r.r = reinterpret_cast<const Region*>(A + off_reg[l]);
+    r.n = (int)lists[l]->v.size();
+    r.nblk = lists[l]->nblk;
+    return r;
+  };
+  const DevPatch* dp = reinterpret_cast<const DevPatch*>(A + off_patch);
+  double* dtp = reinterpret_cast<double*>(A + off_dtp);
+  long* idp = reinterpret_cast<long*>(A + off_idp);
+  Consts C = C0;
+  if (p.problem == MFM_PROBLEM_HELIO) C.r0_dt_ignore = p.sun_r0 - 2.0 * dx;  // define()
+  auto launch = [&](const StageLists& L, const StepArgs& S) -> hipError_t {
+    if (p.precision == MFM_PREC_FP32_FLUX) return launch_stage<float>(C, dp, L, S, dtp, idp, s);
+    return launch_stage<double>(C, dp, L, S, dtp, idp, s);
+  };
+  const double decay_full = C.dedner ? std::exp(-dt * C.ch * C.ch / (C.cp * C.cp)) : 1.0;
+  const double decay_half = C.dedner ? std::exp(-(0.5 * dt) * C.ch * C.ch / (C.cp * C.cp)) : 1.0;
+  if (rk2) {
+    StageLists L1;
+    L1.prim = rl(0); L1.flux[0] = rl(1); L1.flux[1] = rl(2); L1.flux[2] = rl(3); L1.upd = rl(4);
+    for (int d = dims; d < 3; ++d) L1.flux[d].nblk = 0;
+    StepArgs S1{dt, dx, dt + time, decay_full, 0, 1, 0, 0, 0};
+    e = launch(L1, S1);
+    if (e != hipSuccess) return hipfail(c, e, "stage 1 launch");
+    StageLists L2;
+    L2.prim = rl(5); L2.flux[0] = rl(6); L2.flux[1] = rl(7); L2.flux[2] = rl(8); L2.upd = rl(9);
+    for (int d = dims; d < 3; ++d) L2.flux[d].nblk = 0;
+    StepArgs S2{0.5 * dt, dx, 0.5 * dt + time, decay_half, 1, 0, 1, 1, 1};
+    e = launch(L2, S2);
+    if (e != hipSuccess) return hipfail(c, e, "stage 2 launch");
+  } else {
+    StageLists L2;
+    L2.prim = rl(5); L2.flux[0] = rl(6); L2.flux[1] = rl(7); L2.flux[2] = rl(8); L2.upd = rl(9);
+    for (int d = dims; d < 3; ++d) L2.flux[d].nblk = 0;
+    StepArgs S{dt, dx, dt + time, decay_full, 0, 0, 0, 2, 1};
+    e = launch(L2, S);
+    if (e != hipSuccess) return hipfail(c, e, "first-order launch");
+  }
+  RegionList U2 = rl(9);
+  e = launch_dt_final(U2, dtp, idp, reinterpret_cast<double*>(A + off_dto),
+                      reinterpret_cast<long*>(A + off_ido), s);
+  if (e != hipSuccess) return hipfail(c, e, "dt reduction launch");
+  double* hdt = reinterpret_cast<double*>(c->hbuf + table_bytes);
+  long* hid = reinterpret_cast<long*>(c->hbuf + table_bytes + align256(sizeof(double) * n));
+  e = hipMemcpyAsync(hdt, A + off_dto, sizeof(double) * n, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(hid, A + off_ido, sizeof(long) * n, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hipfail(c, e, "patch update");
+  // per-patch results; regions of the last update list are one per patch
+  double best = std::numeric_limits<double>::max();
+  int bestp = -1;
+  bool bad = false;
+  for (int q = 0; q < n; ++q) {
+    const Region& r = st2_upd.v[q];
+    double d = hdt[q];
+    long li = hid[q];
+    int cell[3] = {r.lo[0], r.lo[1], r.lo[2]};
+    if (!(d < std::numeric_limits<double>::infinity())) {
+      d = std::numeric_limits<double>::max();  // every cell excluded
+    } else {
+      long n01 = (long)r.n[0] * r.n[1];
+      cell[2] = r.lo[2] + (int)(li / n01);
+      long rem = li % n01;
+      cell[1] = r.lo[1] + (int)(rem / r.n[0]);
+      cell[0] = r.lo[0] + (int)(rem % r.n[0]);
+    }
+    if (!(d > 0.0) || std::isnan(d)) bad = true;
+    if (new_dt) new_dt[q] = d;
+    if (min_cell) { min_cell[3 * q] = cell[0]; min_cell[3 * q + 1] = cell[1]; min_cell[3 * q + 2] = cell[2]; }
+    if (d < best) { best = d; bestp = q; }
+  }
+  if (level_dt) *level_dt = best;
+  if (min_patch) *min_patch = bestp;
+  if (bad) return fail(c, MFM_ERR_NUMERIC, "non-positive or NaN time step (reference: CH_assert(dt > 0))");
+  return MFM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t mfm_params_sizeof(void) { return sizeof(mfm_params); }
+
+int mfm_params_default(mfm_params* p) {
+  if (!p) return MFM_ERR_ARG;
+  std::memset(p, 0, sizeof(*p));
+  p->abi_version = MFM_ABI_VERSION;
+  p->spacedim = 3;
+  p->nfluids = 1;
+  p->dedner = 0;
+  p->flux = MFM_FLUX_ROE8;                       // main/MHDAM.cpp:911
+  for (int v = 0; v < 8; ++v) p->limiter[v] = MFM_LIM_MINMOD;  // :656 iLimiter = 1
+  p->check_positivity = 0;
+  p->time_approx = MFM_TIME_FIRST_ORDER;         // :1016-1022
+  p->divb_method = 1;                            // :1030
+  p->eight_wave = 1;                             // :1035-1040
+  p->coord_sys = MFM_COORD_CARTESIAN;
+  p->precision = MFM_PREC_FP64;
+  p->entropy_fix = 0.3;                          // :916
+  p->small_b = 1.0e-9;                           // src/RiemannSolver.cpp:42
+  p->gamma = 5.0 / 3.0;
+  p->smallr = 1.0e-8;
+  p->smallp = 1.0e-8;
+  p->dedner_ch = 1.0;                            // srcIdeal/EqSysMHDMF.cpp:17-18
+  p->dedner_cp = 1.0;
+  for (int d = 0; d < 3; ++d) { p->domain_lo[d] = 0; p->domain_hi[d] = 0; p->periodic[d] = 1; }
+  p->problem = MFM_PROBLEM_NONE;
+  p->sun_int_bc_radius = 1.0;
+  p->dir_br_n = 1.0;
+  p->tmlim = 50000.0;
+  p->init_r = 80.0;
+  return MFM_OK;
+}
+
+// SETRIEMANN (problems/RiemannF.ChF:4-63)
+int mfm_params_set_riemann(mfm_params* p, double gamma, const double left[8],
+                           const double right[8], double start_x) {
+  if (!p || !left || !right) return MFM_ERR_ARG;
+  p->problem = MFM_PROBLEM_RIEMANN;
+  p->gamma = gamma;
+  for (int c = 0; c < 8; ++c) { p->rp_left[c] = left[c]; p->rp_right[c] = right[c]; }
+  p->rp_start_x = start_x;
+  const double r0 = std::min(left[0], right[0]);
+  const double p0 = std::min(left[4], right[4]);
+  const double umag = std::sqrt(gamma * p0 / r0);
+  p->smallr = 1.0e-8 * r0;
+  p->smallp = 1.0e-8 * r0 * 0.5 * umag * umag;
+  return MFM_OK;
+}
+
+// SETORSZAG (problems/OrszagF.ChF:4-25)
+int mfm_params_set_orszag(mfm_params* p, double gamma) {
+  if (!p) return MFM_ERR_ARG;
+  p->problem = MFM_PROBLEM_NONE;
+  p->gamma = gamma;
+  const double r0 = gamma * gamma, p0 = gamma;
+  const double umag = std::sqrt(gamma * p0 / r0);
+  p->smallr = 1.0e-8 * r0;
+  p->smallp = 1.0e-8 * r0 * 0.5 * umag * umag;
+  return MFM_OK;
+}
+
+// HeliosphericProblem::input/define (swlism/HeliosphericProblem.cpp:81-330,
+// 463-500), SETHELIOS (swlism/HeliosphericF.ChF:5-131) and
+// SETCHARGEEX_PARS (swlism/ChargeExchange2F.ChF:4-21).
+int mfm_params_set_helio(mfm_params* p, double gamma, double lismN, double lismV,
+                         double lismT, double lismB, const double lismBdir[3],
+                         const double lismUdir[3], const double sunXYZ[3],
+                         double R0, double sunN, double sunV, double sunT,
+                         double sunB, double sunTILT, double initR, double netN,
+                         double TMLIM, double regSafeDZ, double dx_level0,
+                         const int num_cells[3]) {
+  if (!p || !lismBdir || !lismUdir || !sunXYZ || !num_cells) return MFM_ERR_ARG;
+  if (!(dx_level0 > 0.0)) return MFM_ERR_ARG;
+  p->problem = MFM_PROBLEM_HELIO;
+  p->gamma = gamma;
+  p->spacedim = 3;
+  for (int d = 0; d < 3; ++d) {
+    p->domain_lo[d] = 0;
+    p->domain_hi[d] = num_cells[d] - 1;
+    p->periodic[d] = 0;
+  }
+  double sun[3];
+  for (int d = 0; d < 3; ++d) sun[d] = ((int)std::floor(sunXYZ[d] / dx_level0 + 0.5)) * dx_level0;
+  p->sun_xc = sun[0]; p->sun_yc = sun[1]; p->sun_zc = sun[2];
+  const double minZ = -sun[2], maxZ = dx_level0 * num_cells[2] - sun[2];
+  p->reg_safe_z_top = maxZ - regSafeDZ;
+  p->reg_safe_z_bot = minZ + regSafeDZ;
+  p->tmlim = TMLIM;
+  p->sun_r0 = R0;
+  p->init_r = initR;
+  p->sun_int_bc_radius = 1.0;
+  p->dir_br_n = 1.0;
+  p->sun_b_monopolar = 0;
+  p->lism_t = lismT;
+  p->lism_v_dim = lismV;
+  // SETHELIOS
+  const double rho = lismN * eos_mp;
+  const double pref = rho * lismV * lismV;
+  const double pl = 2.0 * eos_k * lismN * lismT;
+  p->lism_p = pl / pref;
+  p->lism_m = 1.0 / std::sqrt(gamma * p->lism_p);
+  const double b = lismB / std::sqrt(pref);
+  p->lism_bx = b * lismBdir[0];
+  p->lism_by = b * lismBdir[1];
+  p->lism_bz = b * lismBdir[2];
+  p->lism_ux = lismUdir[0];
+  p->lism_uy = lismUdir[1];
+  p->lism_uz = lismUdir[2];
+  p->sun_tilt = sunTILT * 0.0174532925199432957692369076848861271;
+  p->omega = 6.28318530717958647692528676655900576 / (25.38 * 24.0 * 3600.0);
+  p->sun_rho = sunN / lismN;
+  p->sun_p = (2.0 * eos_k * sunN * sunT) / pref;
+  p->sun_v = sunV / lismV;
+  p->sun_b = sunB / std::sqrt(pref);
+  if (netN > 0.0) {
+    p->net_rho = netN / lismN;
+    p->net_p = (eos_k * netN * lismT) / pref;
+  }
+  const double r0 = 1.0;
+  const double p0 = 1.0 / (gamma * p->lism_m * p->lism_m);
+  const double umag = std::sqrt(gamma * p0 / r0);
+  p->smallr = 1.0e-8 * r0;
+  p->smallp = 1.0e-8 * r0 * 0.5 * umag * umag;
+  if (p->smallr < 1.0e-12) p->smallr = 1.0e-12;
+  if (p->smallp < 1.0e-12) p->smallp = 1.0e-12;
+  // charge exchange scales, scaleLen = 1 AU
+  p->ce_scale_vel = lismV;
+  p->ce_sigma_lindsay = 1.0e-16 * 1.5e13 * lismN;
+  return MFM_OK;
+}
+
+int mfm_num_components(const mfm_params* p) {
+  if (!p) return MFM_ERR_ARG;
+  return 8 + (p->dedner ? 1 : 0) + 5 * (p->nfluids - 1);
+}
+
+int mfm_num_ghost_cells(const mfm_params* p) {
+  if (!p) return MFM_ERR_ARG;
+  if (p->time_approx == MFM_TIME_RK2) return 4;
+  if (p->time_approx == MFM_TIME_HANCOCK) return 3;
+  return 2;
+}
+
+int mfm_ctx_create(const mfm_params* p, mfm_ctx** out) {
+  if (!out) return MFM_ERR_ARG;
+  *out = nullptr;
+  Consts C;
+  std::string why;
+  int rc = build_consts(p, C, why);
+  if (rc) {
+    std::fprintf(stderr, "mfm_ctx_create: %s\n", why.c_str());
+    return rc;
+  }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) {
+    std::fprintf(stderr, "mfm_ctx_create: no HIP device available\n");
+    return MFM_ERR_DEVICE;
+  }
+  mfm_ctx* c = new mfm_ctx();
+  c->p = *p;
+  c->C = C;
+  (void)hipGetDevice(&c->device);
+  *out = c;
+  return MFM_OK;
+}
+
+int mfm_ctx_destroy(mfm_ctx* c) {
+  if (!c) return MFM_OK;
+  (void)hipDeviceSynchronize();
+  if (c->arena) (void)hipFree(c->arena);
+  if (c->hbuf) (void)hipHostFree(c->hbuf);
+  delete c;
+  return MFM_OK;
+}
+
+int mfm_ctx_set_dedner(mfm_ctx* c, double ch, double cp) {
+  if (!c) return MFM_ERR_ARG;
+  if (!(ch > 0.0) || !(cp > 0.0)) return fail(c, MFM_ERR_ARG, "Ch and Cp must be positive");
+  c->p.dedner_ch = ch;
+  c->p.dedner_cp = cp;
+  c->C.ch = ch;
+  c->C.cp = cp;
+  return MFM_OK;
+}
+
+const char* mfm_ctx_last_error(const mfm_ctx* c) {
+  if (!c) return "null context";
+  return c->err.c_str();
+}
+
+int mfm_patch_update_dev(mfm_ctx* c, mfm_fab U, mfm_box valid, double dt,
+                         double dx, double time, mfm_fab* F, mfm_fab* divB,
+                         double* new_dt, int min_cell[3], void* stream) {
+  if (!c) return MFM_ERR_ARG;
+  PatchIn in{U, valid, F, divB};
+  double d;
+  int cell[3];
+  int rc = run_update(c, &in, 1, dt, dx, time, &d, cell, nullptr, nullptr,
+                      (hipStream_t)stream);
+  if (new_dt) *new_dt = d;
+  if (min_cell) { min_cell[0] = cell[0]; min_cell[1] = cell[1]; min_cell[2] = cell[2]; }
+  return rc;
+}
+
+int mfm_level_update_dev(mfm_ctx* c, const mfm_patch* table, int n, double dt,
+                         double dx, double time, double* new_dt, int* min_cell,
+                         double* level_dt, int* min_patch, void* stream) {
+  if (!c || !table) return MFM_ERR_ARG;
+  std::vector<PatchIn> in(n);
+  for (int q = 0; q < n; ++q) in[q] = PatchIn{table[q].U, table[q].valid, table[q].F, table[q].divB};
+  return run_update(c, in.data(), n, dt, dx, time, new_dt, min_cell, level_dt,
+                    min_patch, (hipStream_t)stream);
+}
+
+int mfm_patch_update_host(mfm_ctx* c, mfm_fab U, mfm_box valid, double dt,
+                          double dx, double time, mfm_fab* F, mfm_fab* divB,
+                          double* new_dt, int min_cell[3]) {
+  if (!c || !U.data) return MFM_ERR_ARG;
+  const int dims = c->C.dims;
+  HBox ub = hb(U.box);
+  size_t ub_bytes = (size_t)ub.size() * U.ncomp * sizeof(double);
+  double* dU = nullptr;
+  std::vector<double*> dF(3, nullptr);
+  double* dD = nullptr;
+  mfm_fab Fd[3];
+  mfm_fab Dd;
+  int rc = MFM_OK;
+  hipError_t e = hipMalloc((void**)&dU, ub_bytes);
+  if (e != hipSuccess) return fail(c, MFM_ERR_NOMEM, "hipMalloc(U)");
+  e = hipMemcpy(dU, U.data, ub_bytes, hipMemcpyHostToDevice);
+  if (F) {
+    for (int d = 0; d < dims && e == hipSuccess; ++d) {
+      size_t b = (size_t)hb(F[d].box).size() * F[d].ncomp * sizeof(double);
+      e = hipMalloc((void**)&dF[d], b);
+      if (e == hipSuccess) e = hipMemset(dF[d], 0, b);
+      Fd[d] = F[d];
+      Fd[d].data = dF[d];
+    }
+  }
+  if (divB && e == hipSuccess) {
+    size_t b = (size_t)hb(divB->box).size() * divB->ncomp * sizeof(double);
+    e = hipMalloc((void**)&dD, b);
+    if (e == hipSuccess) e = hipMemcpy(dD, divB->data, b, hipMemcpyHostToDevice);
+    Dd = *divB;
+    Dd.data = dD;
+  }
+  if (e != hipSuccess) rc = hipfail(c, e, "staging");
+  if (!rc) {
+    mfm_fab Ud = U;
+    Ud.data = dU;
+    rc = mfm_patch_update_dev(c, Ud, valid, dt, dx, time, F ? Fd : nullptr,
+                              divB ? &Dd : nullptr, new_dt, min_cell, nullptr);
+  }
+  if (!rc || rc == MFM_ERR_NUMERIC) {
+    e = hipMemcpy(U.data, dU, ub_bytes, hipMemcpyDeviceToHost);
+    for (int d = 0; F && d < dims && e == hipSuccess; ++d)
+      e = hipMemcpy(F[d].data, dF[d], (size_t)hb(F[d].box).size() * F[d].ncomp * sizeof(double),
+                    hipMemcpyDeviceToHost);
+    if (divB && e == hipSuccess)
+      e = hipMemcpy(divB->data, dD, (size_t)hb(divB->box).size() * divB->ncomp * sizeof(double),
+                    hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hipfail(c, e, "copy back");
+  }
+  (void)hipFree(dU);
+  for (int d = 0; d < 3; ++d)
+    if (dF[d]) (void)hipFree(dF[d]);
+  if (dD) (void)hipFree(dD);
+  return rc;
+}
+
+int mfm_patch_compute_dt_dev(mfm_ctx* c, mfm_fab U, mfm_box box, double dx,
+                             double* dt, int min_cell[3], void* stream) {
+  if (!c || !U.data || !dt) return MFM_ERR_ARG;
+  const Consts& C0 = c->C;
+  if (U.ncomp != C0.nc) return fail(c, MFM_ERR_SHAPE, "U.ncomp mismatch");
+  HBox ub = hb(U.box), b = hb(box);
+  if (!contains(ub, b)) return fail(c, MFM_ERR_SHAPE, "box outside U");
+  RegionBuilder rb;
+  rb.add(0, b);
+  size_t off = 0;
+  size_t off_p = off; off += align256(sizeof(DevPatch));
+  size_t off_r = off; off += align256(sizeof(Region));
+  size_t off_dtp = off; off += align256(sizeof(double) * rb.nblk);
+  size_t off_idp = off; off += align256(sizeof(long) * rb.nblk);
+  size_t off_dto = off; off += align256(sizeof(double));
+  size_t off_ido = off; off += align256(sizeof(long));
+  int rc = ensure_arena(c, off);
+  if (!rc) rc = ensure_hbuf(c, 4096);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  DevPatch* hp = reinterpret_cast<DevPatch*>(c->hbuf);
+  std::memset(hp, 0, sizeof(DevPatch));
+  hp->U = U.data;
+  for (int d = 0; d < 3; ++d) { hp->ulo[d] = ub.lo[d]; hp->un[d] = (int)ub.n(d); }
+  hp->ucs = ub.size();
+  Region* hr = reinterpret_cast<Region*>(c->hbuf + align256(sizeof(DevPatch)));
+  *hr = rb.v[0];
+  char* A = c->arena;
+  hipError_t e = hipMemcpyAsync(A + off_p, c->hbuf, align256(sizeof(DevPatch)) + sizeof(Region),
+                                hipMemcpyHostToDevice, s);
+  Consts C = C0;
+  if (c->p.problem == MFM_PROBLEM_HELIO) C.r0_dt_ignore = c->p.sun_r0 - 2.0 * dx;
+  RegionList L{reinterpret_cast<const Region*>(A + off_r), 1, rb.nblk};
+  if (e == hipSuccess)
+    e = launch_dt_only(C, reinterpret_cast<const DevPatch*>(A + off_p), L, dx,
+                       reinterpret_cast<double*>(A + off_dtp), reinterpret_cast<long*>(A + off_idp), s);
+  if (e == hipSuccess)
+    e = launch_dt_final(L, reinterpret_cast<double*>(A + off_dtp), reinterpret_cast<long*>(A + off_idp),
+                        reinterpret_cast<double*>(A + off_dto), reinterpret_cast<long*>(A + off_ido), s);
+  double* hdt = reinterpret_cast<double*>(c->hbuf + 1024);
+  long* hid = reinterpret_cast<long*>(c->hbuf + 2048);
+  if (e == hipSuccess) e = hipMemcpyAsync(hdt, A + off_dto, sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(hid, A + off_ido, sizeof(long), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hipfail(c, e, "compute_dt");
+  double d = *hdt;
+  const Region& r = rb.v[0];
+  int cell[3] = {r.lo[0], r.lo[1], r.lo[2]};
+  if (!(d < std::numeric_limits<double>::infinity())) {
+    d = std::numeric_limits<double>::max();
+  } else {
+    long n01 = (long)r.n[0] * r.n[1];
+    long li = *hid;
+    cell[2] = r.lo[2] + (int)(li / n01);
+    cell[1] = r.lo[1] + (int)((li % n01) / r.n[0]);
+    cell[0] = r.lo[0] + (int)((li % n01) % r.n[0]);
+  }
+  *dt = d;
+  if (min_cell) { min_cell[0] = cell[0]; min_cell[1] = cell[1]; min_cell[2] = cell[2]; }
+  if (!(d > 0.0)) return fail(c, MFM_ERR_NUMERIC, "non-positive time step");
+  return MFM_OK;
+}
+
+int mfm_init_problem_dev(mfm_ctx* c, mfm_fab U, mfm_box box, double dx,
+                         void* stream) {
+  if (!c || !U.data) return MFM_ERR_ARG;
+  if (U.ncomp != c->C.nc) return fail(c, MFM_ERR_SHAPE, "U.ncomp mismatch");
+  HBox ub = hb(U.box), b = hb(box);
+  if (!contains(ub, b)) return fail(c, MFM_ERR_SHAPE, "box outside U");
+  int ulo[3], un[3], lo[3], n[3];
+  for (int d = 0; d < 3; ++d) { ulo[d] = ub.lo[d]; un[d] = (int)ub.n(d); lo[d] = b.lo[d]; n[d] = (int)b.n(d); }
+  hipError_t e = launch_init(c->C, U.data, ulo, un, ub.size(), lo, n, dx, (hipStream_t)stream);
+  if (e != hipSuccess) return hipfail(c, e, "init launch");
+  return MFM_OK;
+}
+
+int mfm_fill_periodic_dev(mfm_ctx* c, mfm_fab U, mfm_box valid, void* stream) {
+  if (!c || !U.data) return MFM_ERR_ARG;
+  HBox ub = hb(U.box), v = hb(valid);
+  if (!contains(ub, v)) return fail(c, MFM_ERR_SHAPE, "valid outside U");
+  int ulo[3], un[3], vlo[3], vn[3];
+  for (int d = 0; d < 3; ++d) { ulo[d] = ub.lo[d]; un[d] = (int)ub.n(d); vlo[d] = v.lo[d]; vn[d] = (int)v.n(d); }
+  hipError_t e = launch_fill_periodic(U.data, ulo, un, ub.size(), U.ncomp, vlo, vn,
+                                      c->C.dims, (hipStream_t)stream);
+  if (e != hipSuccess) return hipfail(c, e, "periodic fill launch");
+  return MFM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,538 @@
+// mfm_device.hpp -- device-side physics of the multi-fluid MHD patch update.
+//
+// Per-cell / per-face functions for gfx950.  TF is the arithmetic type of
+// reconstruction and Riemann fluxes (double in the parity build, float in the
+// fast build); cons->prim recovery, floors, sources and the state update are
+// always FP64.  Paths below are relative to MHDAM-3.2.4/ of the reference.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+
+#include "../../include/mfm.h"
+
+namespace mfm {
+
+// srcIdeal/EosCommon.fh:13-37
+constexpr double k1_4PI = 0.079577471545947667884441881686257181;
+constexpr double k1_8PI = 0.039788735772973833942220940843128591;
+constexpr double kSQRT_2 = 0.707106781186547524400844362104849039;  // 1/sqrt(2)
+constexpr double kPI = 3.14159265358979323846264338327950288;
+constexpr double k2PI = 6.28318530717958647692528676655900576;
+constexpr double k4PI = 12.566370614359172953850573533118;
+constexpr double kPI_2 = 1.57079632679489661923132169163975144;
+constexpr double kAU = 1.5e13;
+constexpr double kNet2Dens = 1.0e-6, kNet2Pres = 1.0e-8;
+constexpr double kNet3Dens = 1.0e-7, kNet3Pres = 1.0e-9;
+
+constexpr int kMaxComp = 24;  // 8 + 3*5 (+psi for one fluid)
+
+// Constants of one context, passed by value to every kernel (replaces the
+// /eos/, /RiemSol/, /Dedner/, /chargeexcm/ and problem COMMON blocks).
+struct Consts {
+  int dims, nf, nc, dedner, iPsi, iRhoN;
+  int flux_kind;      // 0 Roe8, 1 Rusanov, 2 Dedner (Roe7 + GLM)
+  int iAveraging;
+  int eight_wave, divb_method, problem;
+  int8_t lim[kMaxComp];
+  double gamma, hgamma, invhgamma, smallr, smallp, del2, smallB;
+  double ch, cp;
+  int dom_lo[3], dom_hi[3], phys[3];  // phys[d]: non-periodic physical dir
+  // Riemann problem
+  double rpL[8], rpR[8];
+  // heliospheric
+  double rp_start_x;
+  double sun_xc, sun_yc, sun_zc, sun_r0, sun_rho, sun_p, sun_v, sun_b,
+      sun_tilt, sun_int_bc_radius, dir_br_n, omega, lism_v_dim, init_r;
+  int sun_b_monopolar;
+  double lism_ux, lism_uy, lism_uz, lism_p, lism_m, lism_t, lism_bx, lism_by,
+      lism_bz, net_rho, net_p, tmlim, reg_safe_z_top, reg_safe_z_bot,
+      r0_dt_ignore;
+  double ce_scale_vel, ce_sigma, c13493, c0531;
+  int ce_on;
+};
+
+template <typename T>
+__device__ __forceinline__ T tsqrt(T x) { return sqrt(x); }
+template <typename T>
+__device__ __forceinline__ T tmax(T a, T b) { return a > b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T tmin(T a, T b) { return a < b ? a : b; }
+// Fortran MAX(a, b) with b = floor: returns a if a > b else b
+template <typename T>
+__device__ __forceinline__ T fmaxr(T b, T a) { return a > b ? a : b; }
+
+// In FP32 a rounding error can push a mathematically non-negative radicand
+// below zero; the FP64 build keeps the reference arithmetic untouched.
+template <typename T>
+__device__ __forceinline__ T nonneg(T x) {
+  if constexpr (std::is_same<T, float>::value) return x > 0.f ? x : 0.f;
+  return x;
+}
+
+// ------------------------------------------------------------ limiters --
+// src/reconstructionF.ChF: face values of a cell (left = W-, right = W+)
+// from its neighbours wl, wr along the sweep direction.
+template <typename T>
+__device__ __forceinline__ void limit(int lim, T wl, T w0, T wr, T& left,
+                                      T& right) {
+  const T dWR = wr - w0, dWL = w0 - wl;
+  const T z = T(0), h = T(0.5);
+  switch (lim) {
+    case MFM_LIM_MINMOD: {  // minmod, :126-155
+      T s = copysign(T(1), dWR);
+      T m = s * tmax(z, tmin(fabs(dWR), s * dWL));
+      T sl = h * m;
+      left = w0 - sl; right = w0 + sl;
+      return;
+    }
+    case MFM_LIM_VANLEER: {  // :794-841 (kappa = 1/3)
+      if (dWL * dWR <= z) { left = w0; right = w0; return; }
+      const T BT = T((3.0 - 1.0 / 3.0) / (1.0 - 1.0 / 3.0));
+      const T SL1 = T(0.25 * (1.0 - 1.0 / 3.0)), SL2 = T(0.25 * (1.0 + 1.0 / 3.0));
+      T SL = tmin(fabs(dWR), fabs(BT * dWL));
+      T SR = tmin(fabs(dWL), fabs(BT * dWR));
+      T pl = SL1 * SL + SL2 * SR, pr = SL1 * SR + SL2 * SL;
+      if (dWR < z) { pl = -pl; pr = -pr; }
+      left = w0 - pl; right = w0 + pr;
+      return;
+    }
+    case MFM_LIM_MC: {  // :622-662
+      if (dWL * dWR <= z) { left = w0; right = w0; return; }
+      T AR = fabs(dWR), AL = fabs(dWL), s;
+      if (T(3) * AL <= AR) s = dWL;
+      else if (AL <= T(3) * AR) s = T(0.25) * (dWR + dWL);
+      else s = dWR;
+      left = w0 - s; right = w0 + s;
+      return;
+    }
+    case MFM_LIM_SUPERBEE: {  // :258-300
+      if (dWL * dWR <= z) { left = w0; right = w0; return; }
+      T AR = fabs(dWR), AL = fabs(dWL), s;
+      if (AL <= h * AR) s = dWL;
+      else if (AL <= AR) s = h * dWR;
+      else if (AL <= AR + AR) s = h * dWL;
+      else s = dWR;
+      left = w0 - s; right = w0 + s;
+      return;
+    }
+    case MFM_LIM_VANALBADA: {  // :550-589
+      if (dWL * dWR <= z) { left = w0; right = w0; return; }
+      const T dS = T(1.0e-6), dSS = T(1.0e-6) + T(1.0e-6);
+      T s = h * (dWR * dWL + dS) * (dWR + dWL) / (dWR * dWR + dWL * dWL + dSS);
+      left = w0 - s; right = w0 + s;
+      return;
+    }
+    case MFM_LIM_HARMONIC: {  // :492-524
+      if (dWL * dWR <= z) { left = w0; right = w0; return; }
+      T s = dWR / (T(1) + dWR / dWL);
+      left = w0 - s; right = w0 + s;
+      return;
+    }
+    case MFM_LIM_HYPERBEE: {  // :430-464
+      if (dWL * dWR <= z) { left = w0; right = w0; return; }
+      T A = T(1) + tsqrt(dWR / dWL);
+      T s = dWR / (A * A);
+      s = s + s;
+      left = w0 - s; right = w0 + s;
+      return;
+    }
+    case MFM_LIM_KOREN: {  // :883-918
+      if (dWL * dWR <= z) { left = w0; right = w0; return; }
+      T H = dWL * dWR / (T(2) * (dWL * dWL + dWR * dWR + T(1.0e-10)) - dWL * dWR);
+      left = w0 - (dWL + h * dWR) * H;
+      right = w0 + (dWR + h * dWL) * H;
+      return;
+    }
+    case MFM_LIM_VENKAT: {  // :950-988
+      if (dWL * dWR <= z) { left = w0; right = w0; return; }
+      const T dS = T(1.0e-10), dS2 = T(1.0e-10 * 1.0e-10);
+      T H = (dWR + dWL) / (T(6) * (dWR * dWR + dWL * dWL + dS2));
+      left = w0 - (dWR + T(2) * dWL) * (dWL + dS) * H;
+      right = w0 + (dWL + T(2) * dWR) * (dWR + dS) * H;
+      return;
+    }
+    case MFM_LIM_WENO3: {  // :1096-1136
+      T BR = dWR * dWR + T(1.0e-6), AR = T(1) / (BR * BR);
+      T BL = dWL * dWL + T(1.0e-6), AL = T(1) / (BL * BL);
+      T H = AR / (AR + T(2) * AL);
+      left = w0 - h * (dWL + H * (dWR - dWL));
+      H = AL / (AL + T(2) * AR);
+      right = w0 + h * (dWR - H * (dWR - dWL));
+      return;
+    }
+    case MFM_LIM_WENO3YC: {  // :1169-1212
+      T BR = dWR * dWR + T(1.0e-6), BL = dWL * dWL + T(1.0e-6);
+      T D2P = dWR - dWL;
+      D2P = D2P * D2P;
+      T AL = T(1) + D2P / BL, AR = T(1) + D2P / BR;
+      T H = AR / (AR + T(2) * AL);
+      left = w0 - h * (dWL + H * (dWR - dWL));
+      H = AL / (AL + T(2) * AR);
+      right = w0 + h * (dWR - H * (dWR - dWL));
+      return;
+    }
+    case MFM_LIM_CD:  // :1056-1079 (W- takes the +1 neighbour, as written)
+      left = h * (w0 + wr);
+      right = h * (w0 + wl);
+      return;
+    default:
+      left = w0; right = w0;
+      return;
+  }
+}
+
+// ---------------------------------------------------- cons -> prim (FP64) --
+// CONSTOPRIM / CONSTOPRIM_MF (srcIdeal/PatchIdealMHDF.ChF:143-189,
+// srcIdeal/PatchMHDMFF.ChF:5-80): the multi-fluid version floors rho.
+__device__ __forceinline__ void cons_to_prim(const Consts& C, const double* u,
+                                             double* w) {
+  double rho = u[0];
+  if (C.nf > 1) rho = fmaxr(C.smallr, u[0]);
+  const double ir = 1.0 / rho;
+  const double vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
+  const double ke = u[1] * vx + u[2] * vy + u[3] * vz;
+  const double b = k1_4PI * (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]);
+  double p = C.hgamma * (u[4] - 0.5 * (ke + b));
+  p = fmaxr(C.smallp, p);
+  w[0] = rho; w[1] = vx; w[2] = vy; w[3] = vz; w[4] = p;
+  w[5] = u[5]; w[6] = u[6]; w[7] = u[7];
+  if (C.dedner) w[8] = u[8];
+  for (int f = 1; f < C.nf; ++f) {
+    const int s = C.iRhoN + 5 * (f - 1);
+    const double r = fmaxr(C.smallr, u[s]);
+    const double jr = 1.0 / r;
+    const double ax = u[s + 1] * jr, ay = u[s + 2] * jr, az = u[s + 3] * jr;
+    const double k2 = u[s + 1] * ax + u[s + 2] * ay + u[s + 3] * az;
+    double q = C.hgamma * (u[s + 4] - 0.5 * k2);
+    q = fmaxr(C.smallp, q);
+    w[s] = r; w[s + 1] = ax; w[s + 2] = ay; w[s + 3] = az; w[s + 4] = q;
+  }
+}
+
+// ---------------------------------------------------------- MHD fluxes --
+// Rotated plasma state: p, rho, un, ut1, ut2, bn, bt1, bt2.
+template <typename T>
+struct MState { T p, rho, u, v, w, bx, by, bz; };
+
+// 8-wave Roe flux (RIEMANNF, src/Roe8WaveF.ChF:5-601), left/right already
+// clamped to (smallp, smallr).  Only the non-zero eigenvector entries are
+// formed; the sums keep the reference's term order, so the FP64 build is
+// bit-compatible with the dense form.  out order: rho, mn, mt1, mt2, E, bn,
+// bt1, bt2.  seven = true gives RIEMANN7F (src/Roe7WaveF.ChF:5-558): no B_n
+// wave, out[5] = 0.
+template <typename T, bool seven>
+__device__ __forceinline__ void roe_flux(const Consts& C, const MState<T>& L,
+                                         const MState<T>& R, T* out) {
+  const T half = T(0.5), one = T(1), zero = T(0), two = T(2);
+  const T g = T(C.gamma), hg = T(C.hgamma);
+  const T c4 = T(k1_4PI), c8 = T(k1_8PI), rs2 = T(kSQRT_2);
+
+  const T RUL = L.rho * L.u, RVL = L.rho * L.v, RWL = L.rho * L.w;
+  const T BxBL = c8 * (L.bx * L.bx + L.by * L.by + L.bz * L.bz);
+  const T RUxUL = half * (RUL * L.u + RVL * L.v + RWL * L.w);
+  const T PL = L.p + BxBL;
+  const T EL = L.p / hg + RUxUL + BxBL;
+  const T AL2 = g * L.p / L.rho;
+  const T RUR = R.rho * R.u, RVR = R.rho * R.v, RWR = R.rho * R.w;
+  const T BxBR = c8 * (R.bx * R.bx + R.by * R.by + R.bz * R.bz);
+  const T RUxUR = half * (RUR * R.u + RVR * R.v + RWR * R.w);
+  const T PR = R.p + BxBR;
+  const T ER = R.p / hg + RUxUR + BxBR;
+  const T AR2 = g * R.p / R.rho;
+  T ccmi, ccma;
+  if (AL2 > AR2) { ccmi = AR2; ccma = AL2; } else { ccmi = AL2; ccma = AR2; }
+
+  T Rho, U, V, W, BX, BY, BZ, BX2, BYZ2, BxBH, kk, cc;
+  if (C.iAveraging == 1) {
+    const T HL = (EL + PL) / L.rho, HR = (ER + PR) / R.rho;
+    const T cL = one / (one + tsqrt(R.rho / L.rho));
+    const T cR = one - cL;
+    Rho = R.rho * cL + L.rho * cR;
+    U = L.u * cL + R.u * cR;
+    V = L.v * cL + R.v * cR;
+    W = L.w * cL + R.w * cR;
+    const T HAV = HL * cL + HR * cR;
+    BX = R.bx * cL + L.bx * cR;
+    BY = R.by * cL + L.by * cR;
+    BZ = R.bz * cL + L.bz * cR;
+    const T BXAV = half * (L.bx + R.bx);
+    BX2 = BXAV * BXAV;
+    BYZ2 = BY * BY + BZ * BZ;
+    BxBH = c8 * (BX2 + BYZ2);
+    kk = half * (U * U + V * V + W * W);
+    const T dbx = R.bx - L.bx, dby = R.by - L.by, dbz = R.bz - L.bz;
+    const T X = c8 * (dbx * dbx + dby * dby + dbz * dbz) * cL * cL / L.rho;
+    cc = hg * (HAV - kk - c4 * (BX * BXAV + BYZ2) / Rho) + (two - g) * X;
+  } else {
+    Rho = half * (L.rho + R.rho);
+    U = half * (L.u + R.u); V = half * (L.v + R.v); W = half * (L.w + R.w);
+    BX = half * (L.bx + R.bx); BY = half * (L.by + R.by); BZ = half * (L.bz + R.bz);
+    const T P = half * (PL + PR);
+    BX2 = BX * BX;
+    BYZ2 = BY * BY + BZ * BZ;
+    BxBH = c8 * (BX2 + BYZ2);
+    kk = half * (U * U + V * V + W * W);
+    cc = g * (P - BxBH) / Rho;
+  }
+  const T bb = c4 * BX2 / Rho;
+  const T b = tsqrt(bb);
+  if (cc < ccmi) cc = ccmi;
+  if (cc > ccma) cc = ccma;
+  const T c = tsqrt(cc);
+
+  T hyy, hzz, aaf, aas, af, as;
+  const T sB = T(C.smallB);
+  if (BYZ2 < sB * sB) {
+    hyy = rs2; hzz = rs2;
+    if (bb > cc) { aaf = bb; aas = cc; af = b; as = c; }
+    else { aaf = cc; aas = bb; af = c; as = b; }
+  } else {
+    const T hyz = tsqrt(BYZ2);
+    hyy = BY / hyz; hzz = BZ / hyz;
+    const T szb = cc + (BxBH + BxBH) / Rho;
+    const T hh = tsqrt(nonneg(szb * szb - T(4) * cc * bb));
+    aas = half * (szb - hh);
+    if (aas <= zero) { aas = zero; as = zero; }
+    else if (aas >= cc) { aas = cc; as = c; }
+    else as = tsqrt(aas);
+    aaf = half * (szb + hh);
+    if (aaf <= cc) { aaf = cc; af = c; }
+    else af = tsqrt(aaf);
+  }
+
+  // |lambda| with the entropy fix
+  T A1, A3, A4, A5, A6, A7, A8;  // A2 == A1 (both |U|)
+  {
+    const T eigmax = fabs(U) + fabs(V) + fabs(W) + af;
+    const T del = T(C.del2) * eigmax;
+    const T dl2 = del * del;
+    auto fix = [&](T e) -> T {
+      const T zz = fabs(e);
+      return zz >= del ? zz : half * (zz * zz + dl2) / del;
+    };
+    A1 = fix(U); A3 = fix(U + b); A4 = fix(U - b);
+    A5 = fix(U + af); A6 = fix(U - af); A7 = fix(U + as); A8 = fix(U - as);
+  }
+  const T dfs = aaf - aas;
+  T alf, als;
+  if (dfs < T(1.0e-8)) { alf = one; als = zero; }
+  else if ((cc - aas) <= zero) { alf = zero; als = one; }
+  else if ((aaf - cc) <= zero) { alf = one; als = zero; }
+  else { alf = tsqrt((cc - aas) / dfs); als = tsqrt((aaf - cc) / dfs); }
+  const T sih = (BX >= zero) ? one : -one;
+
+  // differences (left - right) of conserved variables
+  T du1 = L.rho - R.rho, du2 = RUL - RUR, du3 = RVL - RVR, du4 = RWL - RWR;
+  T du5 = EL - ER;
+  const T du6 = L.bx - R.bx, du7 = L.by - R.by, du8 = L.bz - R.bz;
+  if constexpr (!seven) {
+    du5 = -hg * (-kk * du1 + U * du2 + V * du3 + W * du4 - du5 +
+                 (BX * du6 + BY * du7 + BZ * du8) * c4);
+  } else {
+    du5 = -hg * (-kk * du1 + U * du2 + V * du3 + W * du4 - du5 +
+                 (BY * du7 + BZ * du8) * c4);
+  }
+  du2 = (-U * du1 + du2) / Rho;
+  du3 = (-V * du1 + du3) / Rho;
+  du4 = (-W * du1 + du4) / Rho;
+  const T sqPiRho = tsqrt(T(kPI) * Rho), sq2PiRho = tsqrt(T(k2PI) * Rho);
+
+  // non-zero entries of the left/right eigenvector matrices
+  const T ic = one / cc;
+  const T aff = alf * af, ass = als * as;
+  const T hzs = hzz * rs2, hys = hyy * rs2;
+  const T q7 = half * hzz * sih / sq2PiRho, q8 = half * hyy * sih / sq2PiRho;
+  const T csp = c * sqPiRho;
+  // fast/slow rows
+  const T so52 = half * alf * af / cc;
+  const T so53 = -half * als * as * hyy * sih / cc;
+  const T so54 = -half * als * as * hzz * sih / cc;
+  const T so55 = half * alf / (cc * Rho);
+  const T so57 = T(0.25) * als * hyy / csp, so58 = T(0.25) * als * hzz / csp;
+  const T so72 = half * als * as / cc;
+  const T so73 = half * alf * af * hyy * sih / cc;
+  const T so74 = half * alf * af * hzz * sih / cc;
+  const T so75 = half * als / (Rho * cc);
+  const T so77 = -T(0.25) * alf * hyy / csp, so78 = -T(0.25) * alf * hzz / csp;
+
+  const T so15 = -ic;
+  const T lu1 = A1 * (du1 + so15 * du5);
+  const T lu3 = A3 * (-hzs * du3 + hys * du4 + q7 * du7 + (-q8) * du8);
+  const T lu4 = A4 * (-hzs * du3 + hys * du4 + (-q7) * du7 + q8 * du8);
+  const T lu5 = A5 * (so52 * du2 + so53 * du3 + so54 * du4 + so55 * du5 +
+                      so57 * du7 + so58 * du8);
+  const T lu6 = A6 * ((-so52) * du2 + (-so53) * du3 + (-so54) * du4 +
+                      so55 * du5 + so57 * du7 + so58 * du8);
+  const T lu7 = A7 * (so72 * du2 + so73 * du3 + so74 * du4 + so75 * du5 +
+                      so77 * du7 + so78 * du8);
+  const T lu8 = A8 * ((-so72) * du2 + (-so73) * du3 + (-so74) * du4 +
+                      so75 * du5 + so77 * du7 + so78 * du8);
+
+  const T ra_f = Rho * alf, ra_s = Rho * als;
+  const T sp35 = -ass * hyy * sih, sp37 = aff * hyy * sih;
+  const T sp45 = -ass * hzz * sih, sp47 = aff * hzz * sih;
+  const T sp55 = alf * Rho * cc, sp57 = als * Rho * cc;
+  const T sp73 = hzz * sq2PiRho * sih, sp83 = -hyy * sq2PiRho * sih;
+  const T sp75 = two * als * sqPiRho * c * hyy, sp77 = -two * alf * sqPiRho * c * hyy;
+  const T sp85 = two * als * sqPiRho * c * hzz, sp87 = -two * alf * sqPiRho * c * hzz;
+
+  T ra1 = lu1 + ra_f * lu5 + ra_f * lu6 + ra_s * lu7 + ra_s * lu8;
+  T ra2 = aff * lu5 + (-aff) * lu6 + ass * lu7 + (-ass) * lu8;
+  T ra3 = -hzs * lu3 + (-hzs) * lu4 + sp35 * lu5 + (-sp35) * lu6 + sp37 * lu7 +
+          (-sp37) * lu8;
+  T ra4 = hys * lu3 + hys * lu4 + sp45 * lu5 + (-sp45) * lu6 + sp47 * lu7 +
+          (-sp47) * lu8;
+  T ra5 = sp55 * lu5 + sp55 * lu6 + sp57 * lu7 + sp57 * lu8;
+  T ra7 = sp73 * lu3 + (-sp73) * lu4 + sp75 * lu5 + sp75 * lu6 + sp77 * lu7 +
+          sp77 * lu8;
+  T ra8 = sp83 * lu3 + (-sp83) * lu4 + sp85 * lu5 + sp85 * lu6 + sp87 * lu7 +
+          sp87 * lu8;
+  T ra6 = zero;
+  if constexpr (!seven) {
+    ra6 = A1 * du6;  // lu2 = |U| * du6, sp62 = 1
+    ra5 = ra1 * kk + Rho * (U * ra2 + V * ra3 + W * ra4) + ra5 / hg +
+          c4 * (BX * ra6 + BY * ra7 + BZ * ra8);
+  } else {
+    ra5 = ra1 * kk + Rho * (U * ra2 + V * ra3 + W * ra4) + ra5 / hg +
+          c4 * (BY * ra7 + BZ * ra8);
+  }
+  ra2 = U * ra1 + Rho * ra2;
+  ra3 = V * ra1 + Rho * ra3;
+  ra4 = W * ra1 + Rho * ra4;
+
+  const T fl1 = RUL, fr1 = RUR;
+  const T fl2 = L.u * RUL - L.bx * L.bx * c4 + PL;
+  const T fr2 = R.u * RUR - R.bx * R.bx * c4 + PR;
+  const T fl3 = L.u * RVL - L.bx * L.by * c4, fr3 = R.u * RVR - R.bx * R.by * c4;
+  const T fl4 = L.u * RWL - L.bx * L.bz * c4, fr4 = R.u * RWR - R.bx * R.bz * c4;
+  const T fl5 = L.u * (EL + PL) - L.bx * (L.u * L.bx + L.v * L.by + L.w * L.bz) * c4;
+  const T fr5 = R.u * (ER + PR) - R.bx * (R.u * R.bx + R.v * R.by + R.w * R.bz) * c4;
+  const T fl7 = L.u * L.by - L.bx * L.v, fr7 = R.u * R.by - R.bx * R.v;
+  const T fl8 = L.u * L.bz - L.bx * L.w, fr8 = R.u * R.bz - R.bx * R.w;
+  out[0] = half * (fl1 + fr1 + ra1);
+  out[1] = half * (fl2 + fr2 + ra2);
+  out[2] = half * (fl3 + fr3 + ra3);
+  out[3] = half * (fl4 + fr4 + ra4);
+  out[4] = half * (fl5 + fr5 + ra5);
+  out[5] = seven ? zero : half * (zero + zero + ra6);
+  out[6] = half * (fl7 + fr7 + ra7);
+  out[7] = half * (fl8 + fr8 + ra8);
+}
+
+// Rusanov flux (RUSANOVF, src/RusanovF.ChF:4-197)
+template <typename T>
+__device__ __forceinline__ void rusanov_flux(const Consts& C,
+                                             const MState<T>& L,
+                                             const MState<T>& R, T* out) {
+  const T half = T(0.5), one = T(1), two = T(2);
+  const T g = T(C.gamma), hg = T(C.hgamma);
+  const T c4 = T(k1_4PI), c8 = T(k1_8PI);
+  const T RUL = L.rho * L.u, RVL = L.rho * L.v, RWL = L.rho * L.w;
+  const T BxBL = c8 * (L.bx * L.bx + L.by * L.by + L.bz * L.bz);
+  const T RUxUL = half * (RUL * L.u + RVL * L.v + RWL * L.w);
+  const T PL = L.p + BxBL, EL = L.p / hg + RUxUL + BxBL;
+  const T BXL4 = L.bx * c4;
+  const T RUR = R.rho * R.u, RVR = R.rho * R.v, RWR = R.rho * R.w;
+  const T BxBR = c8 * (R.bx * R.bx + R.by * R.by + R.bz * R.bz);
+  const T RUxUR = half * (RUR * R.u + RVR * R.v + RWR * R.w);
+  const T PR = R.p + BxBR, ER = R.p / hg + RUxUR + BxBR;
+  const T BXR4 = R.bx * c4;
+  T Rho, U, BX2, BxBH, A2;
+  if (C.iAveraging == 1) {
+    const T HL = (EL + PL) / L.rho, HR = (ER + PR) / R.rho;
+    const T cL = one / (one + tsqrt(R.rho / L.rho));
+    const T cR = one - cL;
+    Rho = R.rho * cL + L.rho * cR;
+    U = L.u * cL + R.u * cR;
+    const T V = L.v * cL + R.v * cR, W = L.w * cL + R.w * cR;
+    const T HAV = HL * cL + HR * cR;
+    const T BX = R.bx * cL + L.bx * cR, BY = R.by * cL + L.by * cR,
+            BZ = R.bz * cL + L.bz * cR;
+    const T BXAV = half * (L.bx + R.bx);
+    BX2 = BXAV * BXAV;
+    const T BYZ2 = BY * BY + BZ * BZ;
+    BxBH = c8 * (BX2 + BYZ2);
+    const T kk = half * (U * U + V * V + W * W);
+    const T dbx = R.bx - L.bx, dby = R.by - L.by, dbz = R.bz - L.bz;
+    const T X = c8 * (dbx * dbx + dby * dby + dbz * dbz) * cL * cL / L.rho;
+    A2 = hg * (HAV - kk - c4 * (BX * BXAV + BYZ2) / Rho) + (two - g) * X;
+  } else {
+    Rho = half * (L.rho + R.rho);
+    U = half * (L.u + R.u);
+    const T BX = half * (L.bx + R.bx), BY = half * (L.by + R.by),
+            BZ = half * (L.bz + R.bz);
+    const T PGas = half * (L.p + R.p);
+    BX2 = BX * BX;
+    BxBH = c8 * (BX2 + BY * BY + BZ * BZ);
+    A2 = g * PGas / Rho;
+  }
+  const T bb = c4 * BX2 / Rho;
+  const T szb = A2 + (BxBH + BxBH) / Rho;
+  const T hh = tsqrt(nonneg(szb * szb - T(4) * A2 * bb));
+  const T af = tsqrt(half * (szb + hh));
+  const T em = fabs(U) + af;
+  out[0] = half * (RUL + RUR + em * (L.rho - R.rho));
+  out[1] = half * ((L.u * RUL - BXL4 * L.bx + PL) + (R.u * RUR - BXR4 * R.bx + PR) +
+                   em * (RUL - RUR));
+  out[2] = half * ((L.u * RVL - BXL4 * L.by) + (R.u * RVR - BXR4 * R.by) +
+                   em * (RVL - RVR));
+  out[3] = half * ((L.u * RWL - BXL4 * L.bz) + (R.u * RWR - BXR4 * R.bz) +
+                   em * (RWL - RWR));
+  out[4] = half * ((L.u * (EL + PL) - BXL4 * (L.u * L.bx + L.v * L.by + L.w * L.bz)) +
+                   (R.u * (ER + PR) - BXR4 * (R.u * R.bx + R.v * R.by + R.w * R.bz)) +
+                   em * (EL - ER));
+  out[5] = half * (T(0) + T(0) + em * (L.bx - R.bx));
+  out[6] = half * ((L.u * L.by - L.bx * L.v) + (R.u * R.by - R.bx * R.v) +
+                   em * (L.by - R.by));
+  out[7] = half * ((L.u * L.bz - L.bx * L.w) + (R.u * R.bz - R.bx * R.w) +
+                   em * (L.bz - R.bz));
+}
+
+// van Leer flux-vector splitting for one neutral fluid (VANLEERF /
+// VANLEERRS, src/VanLeerRSF.ChF:4-240).  Inputs rotated, already clamped:
+// p, rho, un, ut1, ut2.  out: rho, mn, mt1, mt2, E.
+template <typename T>
+__device__ __forceinline__ void vanleer_flux(const Consts& C, T pL, T rL, T uL,
+                                             T vL, T wL, T pR, T rR, T uR,
+                                             T vR, T wR, T* out) {
+  const T g = T(C.gamma), hg = T(C.hgamma), half = T(0.5), zero = T(0);
+  T p1, p2, p3, p4, p5;  // F+ from the left state
+  {
+    const T RUxU = half * (rL * uL * uL + rL * vL * vL + rL * wL * wL);
+    const T E = pL / hg + RUxU;
+    const T a = tsqrt(g * pL / rL);
+    if (uL <= -a) { p1 = p2 = p3 = p4 = p5 = zero; }
+    else if (uL >= a) {
+      p1 = rL * uL; p2 = p1 * uL + pL; p3 = p1 * vL; p4 = p1 * wL;
+      p5 = uL * (E + pL);
+    } else {
+      const T UA = (-uL + (a + a)) / g;
+      const T H = (E + pL) / rL;
+      const T HU = uL + a;
+      p1 = T(0.25) * rL * HU * HU / a;
+      p2 = p1 * (uL + UA); p3 = p1 * vL; p4 = p1 * wL; p5 = p1 * H;
+    }
+  }
+  T m1, m2, m3, m4, m5;  // F- from the right state
+  {
+    const T RUxU = half * (rR * uR * uR + rR * vR * vR + rR * wR * wR);
+    const T E = pR / hg + RUxU;
+    const T a = tsqrt(g * pR / rR);
+    if (uR >= a) { m1 = m2 = m3 = m4 = m5 = zero; }
+    else if (uR <= -a) {
+      m1 = rR * uR; m2 = m1 * uR + pR; m3 = m1 * vR; m4 = m1 * wR;
+      m5 = uR * (E + pR);
+    } else {
+      const T UA = (-uR - (a + a)) / g;
+      const T H = (E + pR) / rR;
+      const T HU = uR - a;
+      m1 = -T(0.25) * rR * HU * HU / a;
+      m2 = m1 * (uR + UA); m3 = m1 * vR; m4 = m1 * wR; m5 = m1 * H;
+    }
+  }
+  out[0] = m1 + p1; out[1] = m2 + p2; out[2] = m3 + p3; out[3] = m4 + p4;
+  out[4] = m5 + p5;
+}
+
+}  // namespace mfm

@@ -1,0 +1,79 @@
+// mfm_kernels.hpp -- structures shared by the host driver and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mfm {
+
+struct Consts;
+
+// One patch as the kernels see it.  U/Up/W share the index space of the
+// caller's U FAB (box ulo..ulo+un-1); F/Bn scratch live on the face boxes of
+// the first-stage value box.
+struct DevPatch {
+  double* U;        // caller state (updated in place on the valid box)
+  double* Up;       // first-stage state U* (scratch)
+  void* W;          // primitive variables (scratch, flux precision)
+  void* F[3];       // face fluxes per direction (scratch, flux precision)
+  void* Bn[3];      // face-normal B per direction (scratch)
+  double* Fo[3];    // optional time-averaged fluxes returned to the caller
+  double* divB;     // optional div(B) output
+  int ulo[3], un[3];
+  long ucs;         // component stride of U/Up/W
+  int flo[3][3], fn[3][3];
+  long fcs[3];
+  int folo[3][3], fon[3][3];
+  long focs[3];
+  int dlo[3], dn[3];
+  int reinit;       // patch box may intersect the heliospheric inner sphere
+};
+
+// A box of work items belonging to one patch; blk0 = first block.
+struct Region {
+  int patch;
+  int lo[3];
+  int n[3];
+  int blk0;
+};
+
+struct RegionList {
+  const Region* r;  // device pointer
+  int n;            // number of regions
+  int nblk;         // total blocks
+};
+
+struct StageLists {
+  RegionList prim, flux[3], upd;
+};
+
+struct StepArgs {
+  double dt_stage;     // dt of this Euler stage
+  double dx;
+  double time_reinit;  // dt_stage + current time (problem postprocessing)
+  double psi_decay;    // exp(-dt Ch^2 / Cp^2)
+  int in_up;           // stage input: 0 = U, 1 = Up
+  int out_up;          // stage output: 0 = U, 1 = Up
+  int average;         // RK2 second stage: U += (U^n - U*)/2
+  int fo_mode;         // caller fluxes: 0 none, 1 average with stage 1, 2 copy
+  int reduce_dt;       // compute the CFL reduction from the stage output
+};
+
+template <typename TF>
+hipError_t launch_stage(const Consts& C, const DevPatch* P, const StageLists& L,
+                        const StepArgs& A, double* dt_part, long* idx_part,
+                        hipStream_t s);
+
+hipError_t launch_dt_final(const RegionList& L, const double* dt_part,
+                           const long* idx_part, double* dt_out, long* idx_out,
+                           hipStream_t s);
+hipError_t launch_dt_only(const Consts& C, const DevPatch* P, const RegionList& L,
+                          double dx, double* dt_part, long* idx_part,
+                          hipStream_t s);
+hipError_t launch_init(const Consts& C, double* U, const int ulo[3],
+                       const int un[3], long ucs, const int lo[3],
+                       const int n[3], double dx, hipStream_t s);
+hipError_t launch_fill_periodic(double* U, const int ulo[3], const int un[3],
+                                long ucs, int nc, const int vlo[3],
+                                const int vn[3], int dims, hipStream_t s);
+
+}  // namespace mfm

@@ -1,0 +1,27 @@
+// FP64 (parity) build of the stage kernels.  Compiled with
+// -ffp-contract=off so that every product and sum rounds as in the
+// reference's Fortran (no fused multiply-add).
+#define MFM_SHARED_KERNELS 1
+#include "mfm_kernels.inc"
+
+namespace mfm {
+template hipError_t launch_stage<double>(const Consts&, const DevPatch*,
+                                         const StageLists&,
+                                         const StepArgs&, double*, long*,
+                                         hipStream_t);
+
+hipError_t launch_dt_final(const RegionList& L, const double* dt_part,
+                           const long* idx_part, double* dt_out, long* idx_out,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_dt_final, dim3(L.n), dim3(kBlock), 0, s, L.r, L.n,
+                     L.nblk, dt_part, idx_part, dt_out, idx_out);
+  return hipGetLastError();
+}
+hipError_t launch_dt_only(const Consts& C, const DevPatch* P, const RegionList& L,
+                          double dx, double* dt_part, long* idx_part,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_dt_only, dim3(L.nblk), dim3(kBlock), 0, s, C, P, L.r,
+                     L.n, dx, dt_part, idx_part);
+  return hipGetLastError();
+}
+}  // namespace mfm

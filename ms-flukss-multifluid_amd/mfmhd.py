@@ -1,0 +1,299 @@
+"""Python binding of the MI355X patch integrator (libmfmhd.so, include/mfm.h).
+
+Host-side mirror of the reference's patch-integrator interface
+(PatchMHDAM::updateState / computeDt / numGhostCells,
+MHDAM-3.2.4/src/PatchMHDAM.H:252-280) over the C ABI, for tests, benchmarks
+and Python callers.  Device memory comes from torch (plumbing only); every
+computation runs in the HIP kernels of libmfmhd.so.  Loading fails loudly if
+the library has not been built: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmfmhd.so")
+
+# ---- enums (include/mfm.h) ------------------------------------------------
+OK, ERR_ARG, ERR_SHAPE, ERR_DEVICE, ERR_NUMERIC, ERR_NOMEM, ERR_UNSUPPORTED = \
+    0, -1, -2, -3, -4, -5, -6
+FLUX_ROE8, FLUX_RUSANOV, FLUX_ROE8_ROEAVG, FLUX_RUSANOV_ROEAVG = 1, 2, 3, 4
+(LIM_CD, LIM_MINMOD, LIM_SUPERBEE, LIM_VANALBADA, LIM_MC, LIM_HARMONIC,
+ LIM_HYPERBEE, LIM_KOREN, LIM_VENKAT, LIM_VANLEER, LIM_WENO3, LIM_WENO3YC,
+ LIM_NONE) = range(13)
+TIME_FIRST_ORDER, TIME_HANCOCK, TIME_RK2 = 0, 1, 2
+COORD_CARTESIAN, COORD_AXISYMMETRIC = 0, 1
+PROBLEM_NONE, PROBLEM_RIEMANN, PROBLEM_HELIO = 0, 1, 2
+PREC_FP64, PREC_FP32_FLUX = 0, 1
+ABI_VERSION = 1
+
+
+class Box(C.Structure):
+    _fields_ = [("lo", C.c_int * 3), ("hi", C.c_int * 3)]
+
+    @staticmethod
+    def of(lo, hi):
+        b = Box()
+        for d in range(3):
+            b.lo[d] = int(lo[d])
+            b.hi[d] = int(hi[d])
+        return b
+
+    def shape(self):
+        """(nx, ny, nz)"""
+        return tuple(self.hi[d] - self.lo[d] + 1 for d in range(3))
+
+
+class Fab(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("box", Box), ("ncomp", C.c_int)]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_int), ("spacedim", C.c_int), ("nfluids", C.c_int),
+        ("dedner", C.c_int), ("flux", C.c_int), ("limiter", C.c_int * 8),
+        ("check_positivity", C.c_int), ("time_approx", C.c_int),
+        ("divb_method", C.c_int), ("eight_wave", C.c_int),
+        ("coord_sys", C.c_int), ("precision", C.c_int),
+        ("entropy_fix", C.c_double), ("small_b", C.c_double),
+        ("gamma", C.c_double), ("smallr", C.c_double), ("smallp", C.c_double),
+        ("dedner_ch", C.c_double), ("dedner_cp", C.c_double),
+        ("domain_lo", C.c_int * 3), ("domain_hi", C.c_int * 3),
+        ("periodic", C.c_int * 3), ("problem", C.c_int),
+        ("rp_left", C.c_double * 8), ("rp_right", C.c_double * 8),
+        ("rp_start_x", C.c_double),
+        ("sun_xc", C.c_double), ("sun_yc", C.c_double), ("sun_zc", C.c_double),
+        ("sun_r0", C.c_double),
+        ("sun_rho", C.c_double), ("sun_p", C.c_double), ("sun_v", C.c_double),
+        ("sun_b", C.c_double), ("sun_tilt", C.c_double),
+        ("sun_int_bc_radius", C.c_double), ("init_r", C.c_double),
+        ("dir_br_n", C.c_double), ("omega", C.c_double),
+        ("lism_v_dim", C.c_double), ("sun_b_monopolar", C.c_int),
+        ("lism_ux", C.c_double), ("lism_uy", C.c_double),
+        ("lism_uz", C.c_double), ("lism_p", C.c_double), ("lism_m", C.c_double),
+        ("lism_t", C.c_double), ("lism_bx", C.c_double),
+        ("lism_by", C.c_double), ("lism_bz", C.c_double),
+        ("net_rho", C.c_double), ("net_p", C.c_double), ("tmlim", C.c_double),
+        ("reg_safe_z_top", C.c_double), ("reg_safe_z_bot", C.c_double),
+        ("r0_dt_ignore", C.c_double), ("ce_scale_vel", C.c_double),
+        ("ce_sigma_lindsay", C.c_double), ("ce_single_literals", C.c_int),
+    ]
+
+
+class PatchRec(C.Structure):
+    _fields_ = [("U", Fab), ("valid", Box), ("F", C.POINTER(Fab)),
+                ("divB", C.POINTER(Fab))]
+
+
+EXPORTS = [
+    "mfm_params_sizeof", "mfm_params_default", "mfm_params_set_riemann",
+    "mfm_params_set_orszag", "mfm_params_set_helio", "mfm_num_components",
+    "mfm_num_ghost_cells", "mfm_ctx_create", "mfm_ctx_destroy",
+    "mfm_ctx_set_dedner", "mfm_ctx_last_error", "mfm_patch_update_dev",
+    "mfm_patch_update_host", "mfm_patch_compute_dt_dev",
+    "mfm_level_update_dev", "mfm_init_problem_dev", "mfm_fill_periodic_dev",
+]
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load libmfmhd.so (raises if it is missing: no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            "libmfmhd.so not built (%s); run ms-flukss-multifluid_amd/build_lib.py"
+            % path)
+    lib = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    P = C.POINTER(Params)
+    lib.mfm_params_sizeof.restype = C.c_size_t
+    lib.mfm_params_default.argtypes = [P]
+    lib.mfm_params_set_riemann.argtypes = [P, C.c_double, dp, dp, C.c_double]
+    lib.mfm_params_set_orszag.argtypes = [P, C.c_double]
+    lib.mfm_params_set_helio.argtypes = [
+        P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp,
+        dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+        C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+        ip]
+    lib.mfm_num_components.argtypes = [P]
+    lib.mfm_num_ghost_cells.argtypes = [P]
+    lib.mfm_ctx_create.argtypes = [P, C.POINTER(C.c_void_p)]
+    lib.mfm_ctx_destroy.argtypes = [C.c_void_p]
+    lib.mfm_ctx_set_dedner.argtypes = [C.c_void_p, C.c_double, C.c_double]
+    lib.mfm_ctx_last_error.argtypes = [C.c_void_p]
+    lib.mfm_ctx_last_error.restype = C.c_char_p
+    lib.mfm_patch_update_dev.argtypes = [
+        C.c_void_p, Fab, Box, C.c_double, C.c_double, C.c_double,
+        C.POINTER(Fab), C.POINTER(Fab), dp, ip, C.c_void_p]
+    lib.mfm_patch_update_host.argtypes = [
+        C.c_void_p, Fab, Box, C.c_double, C.c_double, C.c_double,
+        C.POINTER(Fab), C.POINTER(Fab), dp, ip]
+    lib.mfm_patch_compute_dt_dev.argtypes = [
+        C.c_void_p, Fab, Box, C.c_double, dp, ip, C.c_void_p]
+    lib.mfm_level_update_dev.argtypes = [
+        C.c_void_p, C.POINTER(PatchRec), C.c_int, C.c_double, C.c_double,
+        C.c_double, dp, ip, dp, ip, C.c_void_p]
+    lib.mfm_init_problem_dev.argtypes = [C.c_void_p, Fab, Box, C.c_double,
+                                         C.c_void_p]
+    lib.mfm_fill_periodic_dev.argtypes = [C.c_void_p, Fab, Box, C.c_void_p]
+    if lib.mfm_params_sizeof() != C.sizeof(Params):
+        raise RuntimeError("mfm_params layout mismatch: C %d vs ctypes %d" %
+                           (lib.mfm_params_sizeof(), C.sizeof(Params)))
+    _lib = lib
+    return lib
+
+
+def _darr(vals):
+    return (C.c_double * len(vals))(*[float(v) for v in vals])
+
+
+def default_params():
+    p = Params()
+    load().mfm_params_default(C.byref(p))
+    return p
+
+
+def riemann_params(gamma, left, right, start_x):
+    p = default_params()
+    load().mfm_params_set_riemann(C.byref(p), gamma, _darr(left), _darr(right),
+                                  start_x)
+    return p
+
+
+def orszag_params(gamma):
+    p = default_params()
+    load().mfm_params_set_orszag(C.byref(p), gamma)
+    return p
+
+
+def helio_params(gamma=5.0 / 3.0, lismN=0.06, lismV=2.64e6, lismT=6527.0,
+                 lismB=1.5e-6, lismBdir=(1.0, 0.0, 0.0),
+                 lismUdir=(-1.0, 0.0, 0.0), sunXYZ=(0.0, 0.0, 0.0), R0=0.1,
+                 sunN=7.4, sunV=4.5e7, sunT=51100.0, sunB=0.0, sunTILT=0.0,
+                 initR=80.0, netN=0.1, TMLIM=50000.0, regSafeDZ=0.0,
+                 dx=1.0, num_cells=(1, 1, 1)):
+    """HeliosphericProblem input with the reference's defaults
+    (MHDAM-3.2.4/swlism/HeliosphericProblem.cpp:83-110)."""
+    p = default_params()
+    nc = (C.c_int * 3)(*[int(v) for v in num_cells])
+    load().mfm_params_set_helio(
+        C.byref(p), gamma, lismN, lismV, lismT, lismB, _darr(lismBdir),
+        _darr(lismUdir), _darr(sunXYZ), R0, sunN, sunV, sunT, sunB, sunTILT,
+        initR, netN, TMLIM, regSafeDZ, dx, nc)
+    return p
+
+
+def num_components(p):
+    return load().mfm_num_components(C.byref(p))
+
+
+def num_ghost_cells(p):
+    return load().mfm_num_ghost_cells(C.byref(p))
+
+
+def fab(data_ptr, lo, hi, ncomp):
+    return Fab(C.c_void_p(int(data_ptr)), Box.of(lo, hi), int(ncomp))
+
+
+class MfmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("mfm error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Context:
+    """A patch integrator bound to the current HIP device (replaces
+    PatchMHDMF construction, define() and the COMMON-block setters)."""
+
+    def __init__(self, params):
+        self.lib = load()
+        self.params = params
+        h = C.c_void_p()
+        rc = self.lib.mfm_ctx_create(C.byref(params), C.byref(h))
+        if rc != OK:
+            raise MfmError(rc, "mfm_ctx_create failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.mfm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            msg = self.lib.mfm_ctx_last_error(self.h)
+            raise MfmError(rc, msg.decode() if msg else "")
+
+    @property
+    def ncomp(self):
+        return num_components(self.params)
+
+    @property
+    def nghost(self):
+        return num_ghost_cells(self.params)
+
+    def set_dedner(self, ch, cp):
+        self._check(self.lib.mfm_ctx_set_dedner(self.h, ch, cp))
+
+    def update_state(self, U, valid, dt, dx, time=0.0, F=None, divB=None,
+                     stream=None, check=True):
+        """PatchMHDAM::updateState on a device FAB; returns (new_dt, cell)."""
+        nd = C.c_double()
+        cell = (C.c_int * 3)()
+        Fp = None
+        if F is not None:
+            arr = (Fab * 3)(*F)
+            Fp = C.cast(arr, C.POINTER(Fab))
+            self._Fkeep = arr
+        dp = C.pointer(divB) if divB is not None else None
+        rc = self.lib.mfm_patch_update_dev(
+            self.h, U, valid, dt, dx, time, Fp, dp, C.byref(nd), cell,
+            C.c_void_p(stream) if stream else None)
+        if check:
+            self._check(rc)
+            return nd.value, tuple(cell)
+        return rc, nd.value, tuple(cell)
+
+    def update_level(self, patches, dt, dx, time=0.0, stream=None):
+        """One launch sequence over a patch table (list of (U, valid)) ->
+        (level_dt, patch index, per-patch dts, per-patch cells)."""
+        n = len(patches)
+        tab = (PatchRec * n)()
+        for q, (U, valid) in enumerate(patches):
+            tab[q].U = U
+            tab[q].valid = valid
+        nd = (C.c_double * n)()
+        mc = (C.c_int * (3 * n))()
+        ld = C.c_double()
+        mp = C.c_int()
+        rc = self.lib.mfm_level_update_dev(
+            self.h, tab, n, dt, dx, time, nd, mc, C.byref(ld), C.byref(mp),
+            C.c_void_p(stream) if stream else None)
+        self._check(rc)
+        return (ld.value, mp.value, list(nd),
+                [tuple(mc[3 * q:3 * q + 3]) for q in range(n)])
+
+    def compute_dt(self, U, box, dx, stream=None):
+        nd = C.c_double()
+        cell = (C.c_int * 3)()
+        self._check(self.lib.mfm_patch_compute_dt_dev(
+            self.h, U, box, dx, C.byref(nd), cell,
+            C.c_void_p(stream) if stream else None))
+        return nd.value, tuple(cell)
+
+    def init_problem(self, U, box, dx, stream=None):
+        self._check(self.lib.mfm_init_problem_dev(
+            self.h, U, box, dx, C.c_void_p(stream) if stream else None))
+
+    def fill_periodic(self, U, valid, stream=None):
+        self._check(self.lib.mfm_fill_periodic_dev(
+            self.h, U, valid, C.c_void_p(stream) if stream else None))

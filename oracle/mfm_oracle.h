@@ -1,0 +1,69 @@
+/*
+ * mfm_oracle.h -- CPU FP64 restatement of the reference patch update.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker (or as the timed CPU baseline).  The product path never calls it.
+ *
+ * Every function here restates a reference routine in the reference's own
+ * operation order (built with -ffp-contract=off, no fast-math); the .cpp
+ * cites the reference file:line for each.
+ */
+#ifndef MFM_ORACLE_H
+#define MFM_ORACLE_H
+#include "../include/mfm.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* PatchMHDAM::updateState on host FABs.  Same contract as
+ * mfm_patch_update_host (include/mfm.h).  nthreads: OpenMP threads (<=0: 1). */
+int oracle_patch_update(const mfm_params* p, double ch, double cp,
+                        mfm_fab U, mfm_box valid, double dt, double dx,
+                        double time, mfm_fab* F, mfm_fab* divB,
+                        double* new_dt, int min_cell[3], int nthreads);
+/* PatchMHDMF::computeDt */
+int oracle_compute_dt(const mfm_params* p, mfm_fab U, mfm_box box, double dx,
+                      double* dt, int min_cell[3]);
+/* One EulerStep (src/PatchMHDAM.cpp:495-709), for stage-level tests. */
+int oracle_euler_step(const mfm_params* p, double ch, double cp, mfm_fab U,
+                      mfm_box box, double dt, double dx, double time,
+                      mfm_fab* F, mfm_fab* divB, int nthreads);
+/* Initial data: HELIOINIT / ORSZAGINIT / RIEMANNINIT over box. */
+int oracle_init_problem(const mfm_params* p, mfm_fab U, mfm_box box, double dx);
+/* Periodic ghost fill for a one-box periodic level. */
+int oracle_fill_periodic(const mfm_params* p, mfm_fab U, mfm_box valid);
+
+/* ---- pointwise kernels, for the pinning tests ------------------------- */
+/* Primitive -> face flux.  wl/wr: plasma primitive states in the face's
+ * rotated order is NOT applied: wl/wr are (rho,u,v,w,p,bx,by,bz[,psi]) in
+ * Cartesian order and idir selects the normal; out receives 8 (9) fluxes. */
+void oracle_flux_roe8(const mfm_params* p, const double* wl, const double* wr,
+                      int idir, int averaging, int lax_friedrichs, double* out);
+void oracle_flux_rusanov(const mfm_params* p, const double* wl,
+                         const double* wr, int idir, int averaging, double* out);
+void oracle_flux_dedner(const mfm_params* p, double ch, const double* wl,
+                        const double* wr, int idir, int averaging, double* out);
+void oracle_flux_vanleer(const mfm_params* p, const double* wl,
+                         const double* wr, int idir, double* out);
+/* 1-D face values of cell 0 from (w_-1, w_0, w_+1): out[0]=W-, out[1]=W+ */
+void oracle_limiter(int lim, double wm, double w0, double wp, double* out);
+/* cons -> prim for one cell (CONSTOPRIM / CONSTOPRIM_MF) */
+void oracle_cons_to_prim(const mfm_params* p, const double* u, double* w);
+/* physical flux (FLUXESHANCOCK, srcIdeal/PatchIdealMHDF.ChF:235-297) */
+void oracle_physical_flux(const mfm_params* p, const double* w, int idir,
+                          double* out);
+/* charge exchange source for one cell (CHARGE_EXCHANGE_4F etc.) */
+void oracle_charge_exchange(const mfm_params* p, const double* w, int reg,
+                            double dt, double* s);
+/* DEFINE_REGIONS_2F for one cell at global index (i,j,k) */
+int oracle_region(const mfm_params* p, const double* w, int i, int j, int k,
+                  double dx);
+/* MAXWAVESPEED + MAXWAVESPEED_E for one cell */
+double oracle_max_wave_speed(const mfm_params* p, const double* u);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,207 @@
+"""ctypes binding of liboracle.so -- the CPU FP64 restatement of the
+reference patch update.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, and only as the checker / the timed CPU
+baseline.  The product (ms-flukss-multifluid_amd/) never imports it.
+
+Arrays are numpy float64 of shape (ncomp, nz, ny, nx), C-contiguous, which is
+exactly the FArrayBox layout (component-major, x fastest).
+"""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "ms-flukss-multifluid_amd"))
+import mfmhd  # noqa: E402  (struct definitions of the C ABI only)
+
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build()
+    lib = C.CDLL(LIB)
+    P = C.POINTER(mfmhd.Params)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    F = C.POINTER(mfmhd.Fab)
+    lib.oracle_patch_update.argtypes = [P, C.c_double, C.c_double, mfmhd.Fab,
+                                        mfmhd.Box, C.c_double, C.c_double,
+                                        C.c_double, F, F, dp, ip, C.c_int]
+    lib.oracle_euler_step.argtypes = [P, C.c_double, C.c_double, mfmhd.Fab,
+                                      mfmhd.Box, C.c_double, C.c_double,
+                                      C.c_double, F, F, C.c_int]
+    lib.oracle_compute_dt.argtypes = [P, mfmhd.Fab, mfmhd.Box, C.c_double, dp, ip]
+    lib.oracle_init_problem.argtypes = [P, mfmhd.Fab, mfmhd.Box, C.c_double]
+    lib.oracle_fill_periodic.argtypes = [P, mfmhd.Fab, mfmhd.Box]
+    lib.oracle_flux_roe8.argtypes = [P, dp, dp, C.c_int, C.c_int, C.c_int, dp]
+    lib.oracle_flux_rusanov.argtypes = [P, dp, dp, C.c_int, C.c_int, dp]
+    lib.oracle_flux_dedner.argtypes = [P, C.c_double, dp, dp, C.c_int, C.c_int, dp]
+    lib.oracle_flux_vanleer.argtypes = [P, dp, dp, C.c_int, dp]
+    lib.oracle_limiter.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp]
+    lib.oracle_cons_to_prim.argtypes = [P, dp, dp]
+    lib.oracle_physical_flux.argtypes = [P, dp, C.c_int, dp]
+    lib.oracle_charge_exchange.argtypes = [P, dp, C.c_int, C.c_double, dp]
+    lib.oracle_region.argtypes = [P, dp, C.c_int, C.c_int, C.c_int, C.c_double]
+    lib.oracle_region.restype = C.c_int
+    lib.oracle_max_wave_speed.argtypes = [P, dp]
+    lib.oracle_max_wave_speed.restype = C.c_double
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def fab_of(arr, lo):
+    """mfmhd.Fab over a numpy array of shape (nc, nz, ny, nx) at lo."""
+    nc, nz, ny, nx = arr.shape
+    hi = (lo[0] + nx - 1, lo[1] + ny - 1, lo[2] + nz - 1)
+    return mfmhd.Fab(C.c_void_p(arr.ctypes.data), mfmhd.Box.of(lo, hi), nc)
+
+
+def patch_update(p, U, lo, valid_lo, valid_hi, dt, dx, time=0.0, ch=1.0,
+                 cp=1.0, want_F=False, want_divB=False, nthreads=1):
+    """PatchMHDAM::updateState on a copy of U; returns a dict."""
+    lib = load()
+    U = np.ascontiguousarray(U, dtype=np.float64).copy()
+    nc = U.shape[0]
+    dims = p.spacedim
+    Fs, Ffabs = [], None
+    if want_F:
+        Ffabs = (mfmhd.Fab * 3)()
+        for d in range(3):
+            flo = list(valid_lo)
+            fhi = list(valid_hi)
+            if d < dims:
+                fhi[d] += 1
+            shp = (nc, fhi[2] - flo[2] + 1, fhi[1] - flo[1] + 1, fhi[0] - flo[0] + 1)
+            a = np.zeros(shp)
+            Fs.append(a)
+            Ffabs[d] = fab_of(a, flo)
+    divB = None
+    dfab = None
+    if want_divB:
+        shp = (1, valid_hi[2] - valid_lo[2] + 1, valid_hi[1] - valid_lo[1] + 1,
+               valid_hi[0] - valid_lo[0] + 1)
+        divB = np.zeros(shp)
+        dfab = fab_of(divB, valid_lo)
+    nd = C.c_double()
+    cell = (C.c_int * 3)()
+    rc = lib.oracle_patch_update(
+        C.byref(p), ch, cp, fab_of(U, lo), mfmhd.Box.of(valid_lo, valid_hi), dt,
+        dx, time, C.cast(Ffabs, C.POINTER(mfmhd.Fab)) if want_F else None,
+        C.pointer(dfab) if want_divB else None, C.byref(nd), cell, nthreads)
+    return {"rc": rc, "U": U, "dt": nd.value, "cell": tuple(cell),
+            "F": Fs[:dims] if want_F else None, "divB": divB}
+
+
+def compute_dt(p, U, lo, box_lo, box_hi, dx):
+    lib = load()
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    nd = C.c_double()
+    cell = (C.c_int * 3)()
+    rc = lib.oracle_compute_dt(C.byref(p), fab_of(U, lo),
+                               mfmhd.Box.of(box_lo, box_hi), dx, C.byref(nd), cell)
+    assert rc == 0
+    return nd.value, tuple(cell)
+
+
+def init_problem(p, shape, lo, box_lo, box_hi, dx):
+    lib = load()
+    U = np.zeros(shape)
+    rc = lib.oracle_init_problem(C.byref(p), fab_of(U, lo),
+                                 mfmhd.Box.of(box_lo, box_hi), dx)
+    assert rc == 0, rc
+    return U
+
+
+def fill_periodic(p, U, lo, valid_lo, valid_hi):
+    lib = load()
+    rc = lib.oracle_fill_periodic(C.byref(p), fab_of(U, lo),
+                                  mfmhd.Box.of(valid_lo, valid_hi))
+    assert rc == 0
+    return U
+
+
+def _vec(x, n=32):
+    a = np.zeros(n)
+    a[:len(x)] = x
+    return a
+
+
+def flux_roe8(p, wl, wr, idir, averaging=0, lf=0):
+    out = np.zeros(32)
+    load().oracle_flux_roe8(C.byref(p), _ptr(_vec(wl)), _ptr(_vec(wr)), idir,
+                            averaging, lf, _ptr(out))
+    return out[:8]
+
+
+def flux_rusanov(p, wl, wr, idir, averaging=0):
+    out = np.zeros(32)
+    load().oracle_flux_rusanov(C.byref(p), _ptr(_vec(wl)), _ptr(_vec(wr)), idir,
+                               averaging, _ptr(out))
+    return out[:8]
+
+
+def flux_dedner(p, ch, wl, wr, idir, averaging=0):
+    out = np.zeros(32)
+    load().oracle_flux_dedner(C.byref(p), ch, _ptr(_vec(wl)), _ptr(_vec(wr)),
+                              idir, averaging, _ptr(out))
+    return out[:9]
+
+
+def flux_vanleer(p, wl, wr, idir):
+    out = np.zeros(32)
+    load().oracle_flux_vanleer(C.byref(p), _ptr(_vec(wl)), _ptr(_vec(wr)), idir,
+                               _ptr(out))
+    return out[:5]
+
+
+def limiter(lim, wm, w0, wp):
+    out = np.zeros(2)
+    load().oracle_limiter(lim, wm, w0, wp, _ptr(out))
+    return out[0], out[1]
+
+
+def cons_to_prim(p, u):
+    w = np.zeros(32)
+    load().oracle_cons_to_prim(C.byref(p), _ptr(_vec(u)), _ptr(w))
+    return w[:len(u)]
+
+
+def physical_flux(p, w, idir):
+    out = np.zeros(32)
+    load().oracle_physical_flux(C.byref(p), _ptr(_vec(w)), idir, _ptr(out))
+    return out[:8]
+
+
+def charge_exchange(p, w, reg, dt):
+    s = np.zeros(32)
+    load().oracle_charge_exchange(C.byref(p), _ptr(_vec(w)), reg, dt, _ptr(s))
+    return s[:len(w)]
+
+
+def region(p, w, i, j, k, dx):
+    return load().oracle_region(C.byref(p), _ptr(_vec(w)), i, j, k, dx)
+
+
+def max_wave_speed(p, u):
+    return load().oracle_max_wave_speed(C.byref(p), _ptr(_vec(u)))

@@ -1,0 +1,287 @@
+"""GPU parity: the HIP patch update (through the C ABI) against the CPU
+oracle on identical inputs.  FP64 build: <= 1e-12 relative (per component,
+normwise) per step; FP32-flux build: per-variable relative L1 <= 1e-5."""
+import numpy as np
+import pytest
+
+import mfmhd
+from helpers import (S4PI, GpuPatch, gpu_update, grid_shape, l1_rel_per_comp,
+                     periodic_fill, rel_err_per_comp, smooth_state, valid_view)
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32_L1 = 1e-5
+
+
+def periodic_params(nf=1, flux=1, lim=1, dims=3, dedner=False, eight=True,
+                    time=mfmhd.TIME_RK2, prec=mfmhd.PREC_FP64, gamma=5.0 / 3.0):
+    p = mfmhd.orszag_params(gamma)
+    p.spacedim = dims
+    p.nfluids = nf
+    p.flux = flux
+    for v in range(8):
+        p.limiter[v] = lim
+    p.dedner = 1 if dedner else 0
+    p.eight_wave = 1 if eight else 0
+    p.time_approx = time
+    p.precision = prec
+    return p
+
+
+def run_pair(oracle_mod, p, U, lo, vlo, vhi, dt, dx, time=0.0, ch=1.0, cp=1.0,
+             want_F=False, want_divB=False):
+    ctx = mfmhd.Context(p)
+    if p.dedner:
+        ctx.set_dedner(ch, cp)
+    g = gpu_update(ctx, U, lo, vlo, vhi, dt, dx, time, want_F, want_divB)
+    o = oracle_mod.patch_update(p, U, lo, vlo, vhi, dt, dx, time, ch=ch, cp=cp,
+                                want_F=want_F, want_divB=want_divB)
+    ctx.close()
+    return g, o
+
+
+def check64(g, o, lo, vlo, vhi, tol=TOL64, U_in=None):
+    assert g["rc"] == 0 and o["rc"] == 0, (g["rc"], o["rc"])
+    a = valid_view(g["U"], lo, vlo, vhi)
+    b = valid_view(o["U"], lo, vlo, vhi)
+    err = rel_err_per_comp(a, b)
+    assert np.all(np.isfinite(a))
+    assert err.max() <= tol, err
+    if U_in is not None:
+        # cells outside the valid box are left unchanged
+        mask = np.ones(U_in.shape[1:], dtype=bool)
+        valid_view(mask[None], lo, vlo, vhi)[...] = False
+        assert np.array_equal(g["U"][:, mask], U_in[:, mask])
+    assert abs(g["dt"] - o["dt"]) <= 1e-12 * o["dt"]
+    assert g["cell"] == o["cell"]
+
+
+CASES = [
+    # (name, nf, flux, limiter)
+    ("roe8-minmod-1f", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_MINMOD),
+    ("roe8roeavg-vanleer-1f", 1, mfmhd.FLUX_ROE8_ROEAVG, mfmhd.LIM_VANLEER),
+    ("rusanov-mc-1f", 1, mfmhd.FLUX_RUSANOV, mfmhd.LIM_MC),
+    ("rusanovavg-superbee-1f", 1, mfmhd.FLUX_RUSANOV_ROEAVG, mfmhd.LIM_SUPERBEE),
+    ("roe8-vanalbada-2f", 2, mfmhd.FLUX_ROE8, mfmhd.LIM_VANALBADA),
+    ("roe8-koren-3f", 3, mfmhd.FLUX_ROE8, mfmhd.LIM_KOREN),
+    ("roe8-weno3-4f", 4, mfmhd.FLUX_ROE8, mfmhd.LIM_WENO3),
+    ("roe8-harmonic-4f", 4, mfmhd.FLUX_ROE8, mfmhd.LIM_HARMONIC),
+    ("roe8-hyperbee-1f", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_HYPERBEE),
+    ("roe8-venkat-1f", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_VENKAT),
+    ("roe8-weno3yc-1f", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_WENO3YC),
+    ("roe8-cd-1f", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_CD),
+]
+
+
+@pytest.mark.parametrize("name,nf,flux,lim", CASES, ids=[c[0] for c in CASES])
+def test_periodic_patch_fp64(oracle_mod, name, nf, flux, lim):
+    print("case", name, flush=True)
+    p = periodic_params(nf=nf, flux=flux, lim=lim)
+    ng = 4
+    vlo, vhi = (0, 0, 0), (15, 13, 11)
+    lo, hi, shape = grid_shape(mfmhd.num_components(p), vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=nf * 7 + flux)
+    periodic_fill(U, ng)
+    dx = 1.0 / 16
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.2 * dx, dx)
+    check64(g, o, lo, vlo, vhi, U_in=U)
+
+
+def test_first_order_time(oracle_mod):
+    p = periodic_params(nf=2, time=mfmhd.TIME_FIRST_ORDER)
+    ng = 2
+    vlo, vhi = (3, -2, 5), (14, 9, 12)
+    lo, hi, shape = grid_shape(mfmhd.num_components(p), vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=3)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.01, 0.05)
+    check64(g, o, lo, vlo, vhi)
+
+
+def test_fluxes_and_divb_outputs(oracle_mod):
+    p = periodic_params(nf=2, flux=mfmhd.FLUX_ROE8, lim=mfmhd.LIM_VANLEER)
+    ng = 4
+    vlo, vhi = (0, 0, 0), (11, 9, 7)
+    lo, hi, shape = grid_shape(mfmhd.num_components(p), vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=11)
+    periodic_fill(U, ng)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.004, 0.05, want_F=True,
+                    want_divB=True)
+    check64(g, o, lo, vlo, vhi)
+    for d in range(3):
+        err = rel_err_per_comp(g["F"][d], o["F"][d])
+        assert err.max() <= TOL64, (d, err)
+    assert rel_err_per_comp(g["divB"], o["divB"]).max() <= 1e-11
+
+
+def test_orszag_tang_dedner_2d(oracle_mod):
+    """Config-2 physics: Orszag-Tang, Dedner-Roe (flux 1 + divB_cleaning 5),
+    8-wave off, periodic, 2-D."""
+    p = periodic_params(nf=1, flux=mfmhd.FLUX_ROE8, lim=mfmhd.LIM_MINMOD,
+                        dims=2, dedner=True, eight=False)
+    n = 32
+    ng = 4
+    vlo, vhi = (0, 0, 0), (n - 1, n - 1, 0)
+    lo, hi, shape = grid_shape(9, vlo, vhi, ng, dims=2)
+    U = oracle_mod.init_problem(p, shape, lo, lo, hi, 1.0 / n)
+    dx = 1.0 / n
+    dtn, _ = oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)
+    ch = 1.0 * dx / dtn
+    cp = np.sqrt(ch * dx / 1.0)
+    # two steps so that psi is non-zero
+    for step in range(2):
+        periodic_fill(U, ng, dims=2)
+        g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.8 * dtn, dx, ch=ch, cp=cp)
+        check64(g, o, lo, vlo, vhi)
+        U = o["U"]
+        dtn = o["dt"]
+        ch = dx / dtn
+        cp = np.sqrt(ch * dx)
+
+
+def test_brio_wu_riemann(oracle_mod):
+    """Config-1 physics: Brio-Wu shock tube with the Riemann problem's
+    ghost fill (RIEMANNGS) and boundary fluxes (FLUXBC)."""
+    L = [1.0, 0, 0, 0, 1.0, 0.75 * S4PI, 1.0 * S4PI, 0]
+    R = [0.125, 0, 0, 0, 0.1, 0.75 * S4PI, -1.0 * S4PI, 0]
+    p = mfmhd.riemann_params(2.0, L, R, 0.5)
+    p.time_approx = mfmhd.TIME_RK2
+    n = 64
+    p.domain_lo[:] = [0, 0, 0]
+    p.domain_hi[:] = [n - 1, 0, 0]
+    p.periodic[:] = [0, 1, 1]
+    ng = 4
+    dx = 1.0 / n
+    # a patch touching the left boundary and one touching the right one
+    for vlo, vhi in (((0, 0, 0), (n - 1, 0, 0)), ((0, 0, 0), (27, 0, 0)),
+                     ((30, 0, 0), (n - 1, 0, 0))):
+        lo, hi, shape = grid_shape(8, vlo, vhi, ng)
+        U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
+        dtn, _ = oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)
+        g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.8 * dtn, dx)
+        check64(g, o, lo, vlo, vhi)
+
+
+def helio_setup(oracle_mod, n=20, nf=4, prec=mfmhd.PREC_FP64):
+    dx = 1.0 / n
+    p = mfmhd.helio_params(lismB=3e-6, lismBdir=(np.cos(np.pi / 4), np.sin(np.pi / 4), 0),
+                           lismV=2.6e6, sunN=7.0, sunV=4.5e7,
+                           sunXYZ=(0.5, 0.5, 0.5), R0=0.12, initR=0.3,
+                           netN=0.1, dx=dx, num_cells=(n, n, n))
+    p.nfluids = nf
+    p.time_approx = mfmhd.TIME_RK2
+    p.precision = prec
+    return p, dx
+
+
+@pytest.mark.parametrize("nf", [4, 2, 1])
+def test_helio_patch(oracle_mod, nf):
+    """Config-3 physics on a small box: HELIOINIT state, LISM inflow and
+    characteristic boundary fluxes on every side, inner-sphere re-init,
+    regions + charge exchange, positivity floors, dt exclusion."""
+    print("helio nf", nf, flush=True)
+    n = 20
+    p, dx = helio_setup(oracle_mod, n, nf)
+    ng = 4
+    vlo, vhi = (0, 0, 0), (n - 1, n - 1, n - 1)
+    nc = mfmhd.num_components(p)
+    lo, hi, shape = grid_shape(nc, vlo, vhi, ng)
+    U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
+    dtn, _ = oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)
+    t = 0.0
+    for step in range(3):
+        g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.5 * dtn, dx, time=t)
+        check64(g, o, lo, vlo, vhi)
+        U = o["U"]
+        t += 0.5 * dtn
+        dtn = o["dt"]
+    # sub-patches: an interior-corner patch and one touching two boundaries
+    for (a, b) in (((3, 4, 2), (11, 12, 9)), ((0, 10, 12), (7, 19, 19))):
+        g, o = run_pair(oracle_mod, p, U, lo, a, b, 0.5 * dtn, dx, time=t)
+        check64(g, o, lo, a, b)
+
+
+def test_level_table_matches_single_patches(oracle_mod):
+    """mfm_level_update_dev over uneven patches == one call per patch."""
+    import torch
+    p = periodic_params(nf=4, flux=mfmhd.FLUX_ROE8, lim=mfmhd.LIM_MINMOD)
+    nc = mfmhd.num_components(p)
+    ng = 4
+    rng = np.random.default_rng(5)
+    boxes = []
+    for q in range(7):
+        e = int(rng.choice([4, 8, 12]))
+        lo0 = rng.integers(-20, 20, size=3)
+        boxes.append((tuple(int(v) for v in lo0), tuple(int(v) + e - 1 for v in lo0)))
+    states = []
+    for q, (vlo, vhi) in enumerate(boxes):
+        lo, hi, shape = grid_shape(nc, vlo, vhi, ng)
+        states.append((smooth_state(p, shape, lo, seed=q), lo, vlo, vhi))
+    ctx = mfmhd.Context(p)
+    singles = []
+    for U, lo, vlo, vhi in states:
+        singles.append(gpu_update(ctx, U, lo, vlo, vhi, 0.01, 0.05))
+    pats = [GpuPatch(U, lo) for U, lo, _, _ in states]
+    ld, mp, dts, cells = ctx.update_level(
+        [(g.fab, mfmhd.Box.of(s[2], s[3])) for g, s in zip(pats, states)], 0.01, 0.05)
+    torch.cuda.synchronize()
+    for q, (g, s) in enumerate(zip(pats, singles)):
+        assert np.array_equal(g.numpy(), s["U"]), q
+        assert dts[q] == s["dt"] and cells[q] == s["cell"]
+    assert ld == min(dts) and dts[mp] == ld
+    # and each single equals the oracle
+    U, lo, vlo, vhi = states[2]
+    o = oracle_mod.patch_update(p, U, lo, vlo, vhi, 0.01, 0.05)
+    check64(singles[2], o, lo, vlo, vhi)
+    ctx.close()
+
+
+def test_compute_dt(oracle_mod):
+    p = periodic_params(nf=3)
+    ng = 4
+    vlo, vhi = (2, 1, 0), (13, 10, 8)
+    lo, hi, shape = grid_shape(mfmhd.num_components(p), vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=4)
+    ctx = mfmhd.Context(p)
+    g = GpuPatch(U, lo)
+    d, cell = ctx.compute_dt(g.fab, mfmhd.Box.of(vlo, vhi), 0.05)
+    od, ocell = oracle_mod.compute_dt(p, U, lo, vlo, vhi, 0.05)
+    assert abs(d - od) <= 1e-13 * od and cell == ocell
+    ctx.close()
+
+
+@pytest.mark.parametrize("nf", [1, 4])
+def test_fp32_flux_path_error(oracle_mod, nf):
+    """FP32 reconstruction + fluxes, FP64 recovery/floors/accumulation:
+    per-variable relative L1 error against the FP64 oracle <= 1e-5."""
+    p, dx = helio_setup(oracle_mod, 20, nf, prec=mfmhd.PREC_FP32_FLUX)
+    ng = 4
+    vlo, vhi = (0, 0, 0), (19, 19, 19)
+    nc = mfmhd.num_components(p)
+    lo, hi, shape = grid_shape(nc, vlo, vhi, ng)
+    U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
+    dtn, _ = oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.5 * dtn, dx)
+    assert g["rc"] == 0
+    err = l1_rel_per_comp(valid_view(g["U"], lo, vlo, vhi), valid_view(o["U"], lo, vlo, vhi))
+    print("fp32 per-variable L1:", " ".join("%.2e" % e for e in err))
+    assert err.max() <= TOL32_L1, err
+
+
+def test_shape_errors_leave_state_untouched():
+    import torch
+    p = periodic_params(nf=1)
+    ctx = mfmhd.Context(p)
+    U = np.ones((8, 10, 10, 10))
+    g = GpuPatch(U, (0, 0, 0))
+    # valid box grown by 4 ghosts is not inside U
+    rc, _, _ = ctx.update_state(g.fab, mfmhd.Box.of((1, 1, 1), (8, 8, 8)), 0.1, 0.1,
+                                check=False)
+    assert rc == mfmhd.ERR_SHAPE
+    bad = mfmhd.fab(g.t.data_ptr(), (0, 0, 0), (9, 9, 9), 9)
+    rc, _, _ = ctx.update_state(bad, mfmhd.Box.of((4, 4, 4), (5, 5, 5)), 0.1, 0.1,
+                                check=False)
+    assert rc == mfmhd.ERR_SHAPE
+    torch.cuda.synchronize()
+    assert np.array_equal(g.numpy(), U)
+    ctx.close()
