@@ -93,8 +93,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
-    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    assert mfmhd.device_count() > 0, "bench.py needs a HIP device"
     prec = mfmhd.PREC_FP32_FLUX if args.precision == "fp32flux" else mfmhd.PREC_FP64
     n = args.n
     p, dx = c3_params(n, prec)
@@ -103,8 +102,8 @@ def main():
     ng = ctx.nghost
     lo, hi = (-ng,) * 3, (n - 1 + ng,) * 3
     m = n + 2 * ng
-    U = torch.empty((nc, m, m, m), dtype=torch.float64, device="cuda")
-    Uf = mfmhd.fab(U.data_ptr(), lo, hi, nc)
+    U = mfmhd.DeviceArray((nc, m, m, m))
+    Uf = mfmhd.fab(U.ptr, lo, hi, nc)
     ctx.init_problem(Uf, mfmhd.Box.of(lo, hi), dx)
     valid = mfmhd.Box.of((0, 0, 0), (n - 1,) * 3)
     dtn, _ = ctx.compute_dt(Uf, valid, dx)
@@ -113,9 +112,8 @@ def main():
         dt = args.cfl * dtn
         dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
         t += dt
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
+    mfmhd.synchronize()
+    e0, e1 = mfmhd.Event(), mfmhd.Event()
     e0.record()
     w0 = time.perf_counter()
     for _ in range(args.steps):
@@ -123,9 +121,9 @@ def main():
         dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
         t += dt
     e1.record()
-    torch.cuda.synchronize()
+    mfmhd.synchronize()
     wall = time.perf_counter() - w0
-    ms = e0.elapsed_time(e1) / args.steps
+    ms = e0.elapsed_ms(e1) / args.steps
     cells = float(n) ** 3
     value = cells / (ms * 1e-3)
     alg_bytes = 5 * nc * 8  # per cell-update (RK2: read U, write U*, read U*, U, write U)
@@ -154,8 +152,7 @@ def main():
         "wall_s_timed": wall,
     }
     ctx.close()
-    del U
-    torch.cuda.empty_cache()
+    U.free()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cb = cpu_baseline(args.cpu_sample, threads)

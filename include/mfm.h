@@ -220,6 +220,20 @@ int mfm_init_problem_dev(mfm_ctx* ctx, mfm_fab U, mfm_box box, double dx,
  * LevelData::exchange that a one-box level needs).                         */
 int mfm_fill_periodic_dev(mfm_ctx* ctx, mfm_fab U, mfm_box valid, void* stream);
 
+/* ---- device utilities for FFI callers without their own HIP runtime ----
+ * (allocation, copies and stream-ordered timing on the library's runtime). */
+int mfm_dev_count(void);
+int mfm_dev_alloc(size_t bytes, void** ptr);
+int mfm_dev_free(void* ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device (synchronous)   */
+int mfm_dev_memcpy(void* dst, const void* src, size_t bytes, int kind);
+int mfm_dev_memset(void* ptr, int value, size_t bytes);
+int mfm_dev_synchronize(void);
+int mfm_event_create(void** ev);
+int mfm_event_destroy(void* ev);
+int mfm_event_record(void* ev, void* stream);
+int mfm_event_elapsed_ms(void* start, void* stop, float* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -817,3 +817,65 @@ int mfm_fill_periodic_dev(mfm_ctx* c, mfm_fab U, mfm_box valid, void* stream) {
 }
 
 }  // extern "C"
+
+// ---- device utilities ------------------------------------------------------
+extern "C" {
+
+int mfm_dev_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int mfm_dev_alloc(size_t bytes, void** ptr) {
+  if (!ptr) return MFM_ERR_ARG;
+  *ptr = nullptr;
+  if (hipMalloc(ptr, bytes) != hipSuccess) return MFM_ERR_NOMEM;
+  return MFM_OK;
+}
+
+int mfm_dev_free(void* ptr) {
+  if (!ptr) return MFM_OK;
+  return hipFree(ptr) == hipSuccess ? MFM_OK : MFM_ERR_DEVICE;
+}
+
+int mfm_dev_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                    : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  if (kind < 0 || kind > 2) return MFM_ERR_ARG;
+  return hipMemcpy(dst, src, bytes, k) == hipSuccess ? MFM_OK : MFM_ERR_DEVICE;
+}
+
+int mfm_dev_memset(void* ptr, int value, size_t bytes) {
+  return hipMemset(ptr, value, bytes) == hipSuccess ? MFM_OK : MFM_ERR_DEVICE;
+}
+
+int mfm_dev_synchronize(void) {
+  return hipDeviceSynchronize() == hipSuccess ? MFM_OK : MFM_ERR_DEVICE;
+}
+
+int mfm_event_create(void** ev) {
+  if (!ev) return MFM_ERR_ARG;
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return MFM_ERR_DEVICE;
+  *ev = (void*)e;
+  return MFM_OK;
+}
+
+int mfm_event_destroy(void* ev) {
+  return hipEventDestroy((hipEvent_t)ev) == hipSuccess ? MFM_OK : MFM_ERR_DEVICE;
+}
+
+int mfm_event_record(void* ev, void* stream) {
+  return hipEventRecord((hipEvent_t)ev, (hipStream_t)stream) == hipSuccess ? MFM_OK
+                                                                          : MFM_ERR_DEVICE;
+}
+
+int mfm_event_elapsed_ms(void* start, void* stop, float* ms) {
+  if (!ms) return MFM_ERR_ARG;
+  if (hipEventSynchronize((hipEvent_t)stop) != hipSuccess) return MFM_ERR_DEVICE;
+  return hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop) == hipSuccess
+             ? MFM_OK : MFM_ERR_DEVICE;
+}
+
+}  // extern "C"

@@ -3,9 +3,10 @@
 Host-side mirror of the reference's patch-integrator interface
 (PatchMHDAM::updateState / computeDt / numGhostCells,
 MHDAM-3.2.4/src/PatchMHDAM.H:252-280) over the C ABI, for tests, benchmarks
-and Python callers.  Device memory comes from torch (plumbing only); every
-computation runs in the HIP kernels of libmfmhd.so.  Loading fails loudly if
-the library has not been built: there is no CPU fallback.
+and Python callers.  Device memory, copies and timing go through the
+library's own HIP runtime (DeviceArray, Event), so a process holds one HIP
+runtime; every computation runs in the HIP kernels of libmfmhd.so.  Loading
+fails loudly if the library has not been built: there is no CPU fallback.
 """
 import ctypes as C
 import os
@@ -91,6 +92,9 @@ EXPORTS = [
     "mfm_ctx_set_dedner", "mfm_ctx_last_error", "mfm_patch_update_dev",
     "mfm_patch_update_host", "mfm_patch_compute_dt_dev",
     "mfm_level_update_dev", "mfm_init_problem_dev", "mfm_fill_periodic_dev",
+    "mfm_dev_count", "mfm_dev_alloc", "mfm_dev_free", "mfm_dev_memcpy",
+    "mfm_dev_memset", "mfm_dev_synchronize", "mfm_event_create",
+    "mfm_event_destroy", "mfm_event_record", "mfm_event_elapsed_ms",
 ]
 
 _lib = None
@@ -139,6 +143,15 @@ def load(path=LIB_PATH):
     lib.mfm_init_problem_dev.argtypes = [C.c_void_p, Fab, Box, C.c_double,
                                          C.c_void_p]
     lib.mfm_fill_periodic_dev.argtypes = [C.c_void_p, Fab, Box, C.c_void_p]
+    lib.mfm_dev_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+    lib.mfm_dev_free.argtypes = [C.c_void_p]
+    lib.mfm_dev_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    lib.mfm_dev_memset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    lib.mfm_event_create.argtypes = [C.POINTER(C.c_void_p)]
+    lib.mfm_event_destroy.argtypes = [C.c_void_p]
+    lib.mfm_event_record.argtypes = [C.c_void_p, C.c_void_p]
+    lib.mfm_event_elapsed_ms.argtypes = [C.c_void_p, C.c_void_p,
+                                         C.POINTER(C.c_float)]
     if lib.mfm_params_sizeof() != C.sizeof(Params):
         raise RuntimeError("mfm_params layout mismatch: C %d vs ctypes %d" %
                            (lib.mfm_params_sizeof(), C.sizeof(Params)))
@@ -297,3 +310,90 @@ class Context:
     def fill_periodic(self, U, valid, stream=None):
         self._check(self.lib.mfm_fill_periodic_dev(
             self.h, U, valid, C.c_void_p(stream) if stream else None))
+
+
+# ---- device memory and timing on the library's runtime ---------------------
+def device_count():
+    return load().mfm_dev_count()
+
+
+def synchronize():
+    rc = load().mfm_dev_synchronize()
+    if rc != OK:
+        raise MfmError(rc, "device synchronize failed")
+
+
+class DeviceArray:
+    """A float64 device buffer holding a numpy-shaped array."""
+
+    def __init__(self, shape, data=None, zero=False):
+        import numpy as np
+        self.lib = load()
+        self.shape = tuple(int(s) for s in shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        p = C.c_void_p()
+        rc = self.lib.mfm_dev_alloc(max(self.nbytes, 8), C.byref(p))
+        if rc != OK:
+            raise MfmError(rc, "device allocation of %d bytes failed" % self.nbytes)
+        self.ptr = p.value
+        if data is not None:
+            self.upload(data)
+        elif zero:
+            self.lib.mfm_dev_memset(C.c_void_p(self.ptr), 0, self.nbytes)
+
+    def upload(self, a):
+        import numpy as np
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == self.shape, (a.shape, self.shape)
+        rc = self.lib.mfm_dev_memcpy(C.c_void_p(self.ptr), C.c_void_p(a.ctypes.data),
+                                     self.nbytes, 0)
+        if rc != OK:
+            raise MfmError(rc, "host->device copy failed")
+
+    def numpy(self):
+        import numpy as np
+        a = np.empty(self.shape, dtype=np.float64)
+        rc = self.lib.mfm_dev_memcpy(C.c_void_p(a.ctypes.data), C.c_void_p(self.ptr),
+                                     self.nbytes, 1)
+        if rc != OK:
+            raise MfmError(rc, "device->host copy failed")
+        return a
+
+    def free(self):
+        if self.ptr:
+            self.lib.mfm_dev_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    """A HIP event for stream-ordered timing."""
+
+    def __init__(self):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.mfm_event_create(C.byref(h))
+        if rc != OK:
+            raise MfmError(rc, "event create failed")
+        self.h = h
+
+    def record(self, stream=None):
+        self.lib.mfm_event_record(self.h, C.c_void_p(stream) if stream else None)
+
+    def elapsed_ms(self, stop):
+        ms = C.c_float()
+        rc = self.lib.mfm_event_elapsed_ms(self.h, stop.h, C.byref(ms))
+        if rc != OK:
+            raise MfmError(rc, "event timing failed")
+        return ms.value
+
+    def __del__(self):
+        try:
+            self.lib.mfm_event_destroy(self.h)
+        except Exception:
+            pass

@@ -1321,7 +1321,7 @@ int region_cell(const Eq& E, const double* w, int i, int j, int k, double dx) {
 
 // Charge exchange, CHARGE_EXCHANGE_2F/_3F/_4F (swlism/ChargeExchange2F.ChF:
 // 294-432, 434-671, 673-993): plasma sink summed over neutrals in order,
-// each neutral born in its region (1|4 -> 1, 2 -> 2, 3 -> 3) or lost.
+// each neutral born in its region or lost.
 void charge_exchange_cell(const Eq& E, const double* w, int REG, double dt,
                           double* s) {
   const mfm_params* P = E.P;
@@ -1392,7 +1392,9 @@ void charge_exchange_cell(const Eq& E, const double* w, int REG, double dt,
       PMI[d] = ARR[f] * (un[f][d] * UD[f] + (un[f][d] - uu[d]) * VVTHU[f]);
     double ENMI = ARR[f] * (0.5 * UH2[f] * UD[f] - VPHH[f] * VVTHU[f] +
                             0.75 * VVTH[f] * UPE[f]);
-    bool born = (f == 0) ? (REG == 1 || REG == 4) : (REG == f + 1);
+    // neutral 1 is born in region 1 (and 4 in the 2- and 4-fluid kernels,
+    // CHARGE_EXCHANGE_3F tests REG == 1 only, :618); neutral k in region k
+    bool born = (f == 0) ? (REG == 1 || (REG == 4 && E.nf != 3)) : (REG == f + 1);
     double H1, H2, H3, H4, H5;
     if (born) {
       // sum of the other fluids' DENPLU in index order (zero if none)

@@ -15,11 +15,8 @@ def pytest_configure(config):
 
 
 def gpu_available():
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
+    import mfmhd
+    return mfmhd.device_count() > 0
 
 
 @pytest.fixture(scope="session")
