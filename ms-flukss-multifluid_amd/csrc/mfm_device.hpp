@@ -27,6 +27,17 @@ constexpr double kNet3Dens = 1.0e-7, kNet3Pres = 1.0e-9;
 
 constexpr int kMaxComp = 24;  // 8 + 3*5 (+psi for one fluid)
 
+// Compile-time equation system: NF fluids (plasma + NF-1 neutrals), DED = 1
+// adds the GLM psi component.  Every kernel is instantiated per system so
+// all component loops unroll and per-cell vectors live in registers.
+template <int NF_, int DED_>
+struct Sys {
+  static constexpr int NF = NF_;
+  static constexpr int DED = DED_;
+  static constexpr int NC = 8 + DED_ + 5 * (NF_ - 1);
+  static constexpr int IRN = 8 + DED_;  // first neutral component
+};
+
 // Constants of one context, passed by value to every kernel (replaces the
 // /eos/, /RiemSol/, /Dedner/, /chargeexcm/ and problem COMMON blocks).
 struct Consts {
@@ -185,10 +196,11 @@ __device__ __forceinline__ void limit(int lim, T wl, T w0, T wr, T& left,
 // ---------------------------------------------------- cons -> prim (FP64) --
 // CONSTOPRIM / CONSTOPRIM_MF (srcIdeal/PatchIdealMHDF.ChF:143-189,
 // srcIdeal/PatchMHDMFF.ChF:5-80): the multi-fluid version floors rho.
+template <class S>
 __device__ __forceinline__ void cons_to_prim(const Consts& C, const double* u,
                                              double* w) {
   double rho = u[0];
-  if (C.nf > 1) rho = fmaxr(C.smallr, u[0]);
+  if (S::NF > 1) rho = fmaxr(C.smallr, u[0]);
   const double ir = 1.0 / rho;
   const double vx = u[1] * ir, vy = u[2] * ir, vz = u[3] * ir;
   const double ke = u[1] * vx + u[2] * vy + u[3] * vz;
@@ -197,9 +209,10 @@ __device__ __forceinline__ void cons_to_prim(const Consts& C, const double* u,
   p = fmaxr(C.smallp, p);
   w[0] = rho; w[1] = vx; w[2] = vy; w[3] = vz; w[4] = p;
   w[5] = u[5]; w[6] = u[6]; w[7] = u[7];
-  if (C.dedner) w[8] = u[8];
-  for (int f = 1; f < C.nf; ++f) {
-    const int s = C.iRhoN + 5 * (f - 1);
+  if (S::DED) w[8] = u[8];
+#pragma unroll
+  for (int f = 1; f < S::NF; ++f) {
+    const int s = S::IRN + 5 * (f - 1);
     const double r = fmaxr(C.smallr, u[s]);
     const double jr = 1.0 / r;
     const double ax = u[s + 1] * jr, ay = u[s + 2] * jr, az = u[s + 3] * jr;

@@ -20,8 +20,18 @@ hipError_t launch_dt_final(const RegionList& L, const double* dt_part,
 hipError_t launch_dt_only(const Consts& C, const DevPatch* P, const RegionList& L,
                           double dx, double* dt_part, long* idx_part,
                           hipStream_t s) {
-  hipLaunchKernelGGL(k_dt_only, dim3(L.nblk), dim3(kBlock), 0, s, C, P, L.r,
-                     L.n, dx, dt_part, idx_part);
+#define MFM_DT_ONLY(NF, DED)                                                    \
+  hipLaunchKernelGGL((k_dt_only<Sys<NF, DED>>), dim3(L.nblk), dim3(kBlock), 0, \
+                     s, C, P, L.r, L.n, dx, dt_part, idx_part)
+  switch (C.nf * 2 + C.dedner) {
+    case 2: MFM_DT_ONLY(1, 0); break;
+    case 3: MFM_DT_ONLY(1, 1); break;
+    case 4: MFM_DT_ONLY(2, 0); break;
+    case 6: MFM_DT_ONLY(3, 0); break;
+    case 8: MFM_DT_ONLY(4, 0); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef MFM_DT_ONLY
   return hipGetLastError();
 }
 }  // namespace mfm
