@@ -282,30 +282,51 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
     if (P.divB && (!P.divB->data || P.divB->ncomp < 1))
       return fail(c, MFM_ERR_SHAPE, "divB must have one component");
   }
-  // region lists
-  RegionBuilder st1_prim, st1_flux[3], st1_upd, st2_prim, st2_flux[3], st2_upd;
+  // region lists; faces on a physical boundary go to their own list
+  RegionBuilder st1_prim, st1_flux[3], st1_fbc[3], st1_upd, st2_prim, st2_flux[3],
+      st2_fbc[3], st2_upd;
+  auto add_faces = [&](RegionBuilder& in, RegionBuilder& bc, int q, HBox f, int d) {
+    if (C0.phys[d]) {
+      if (f.lo[d] == C0.dom_lo[d]) {
+        HBox b = f;
+        b.hi[d] = b.lo[d];
+        bc.add(q, b);
+        f.lo[d] += 1;
+      }
+      if (f.hi[d] == C0.dom_hi[d] + 1 && f.hi[d] >= f.lo[d]) {
+        HBox b = f;
+        b.lo[d] = b.hi[d];
+        bc.add(q, b);
+        f.hi[d] -= 1;
+      }
+    }
+    if (f.hi[d] >= f.lo[d]) in.add(q, f);
+  };
   for (int q = 0; q < n; ++q) {
     const PatchGeo& g = G[q];
     if (rk2) {
       st1_prim.add(q, grow(g.vb1, 2, dims));
-      for (int d = 0; d < dims; ++d) st1_flux[d].add(q, g.fb[d]);
+      for (int d = 0; d < dims; ++d) add_faces(st1_flux[d], st1_fbc[d], q, g.fb[d], d);
       st1_upd.add(q, g.vb1);
     }
     st2_prim.add(q, grow(g.vb2, 2, dims));
     for (int d = 0; d < dims; ++d) {
       HBox f = g.vb2;
       f.hi[d] += 1;
-      st2_flux[d].add(q, f);
+      add_faces(st2_flux[d], st2_fbc[d], q, f, d);
     }
     st2_upd.add(q, g.vb2);
   }
-  RegionBuilder* lists[10] = {&st1_prim, &st1_flux[0], &st1_flux[1], &st1_flux[2], &st1_upd,
-                              &st2_prim, &st2_flux[0], &st2_flux[1], &st2_flux[2], &st2_upd};
+  constexpr int kLists = 16;
+  RegionBuilder* lists[kLists] = {&st1_prim, &st1_flux[0], &st1_flux[1], &st1_flux[2],
+                                  &st1_fbc[0], &st1_fbc[1], &st1_fbc[2], &st1_upd,
+                                  &st2_prim, &st2_flux[0], &st2_flux[1], &st2_flux[2],
+                                  &st2_fbc[0], &st2_fbc[1], &st2_fbc[2], &st2_upd};
   // table layout inside the arena (after the scratch)
   size_t off_patch = off; off += align256(sizeof(DevPatch) * n);
-  size_t off_reg[10];
+  size_t off_reg[kLists];
   size_t table_bytes = align256(sizeof(DevPatch) * n);
-  for (int l = 0; l < 10; ++l) {
+  for (int l = 0; l < kLists; ++l) {
     off_reg[l] = off;
     size_t b = align256(sizeof(Region) * std::max<size_t>(1, lists[l]->v.size()));
     off += b;
@@ -368,7 +389,7 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
     }
   }
   size_t hoff = align256(sizeof(DevPatch) * n);
-  for (int l = 0; l < 10; ++l) {
+  for (int l = 0; l < kLists; ++l) {
     std::memcpy(h + hoff, lists[l]->v.data(), sizeof(Region) * lists[l]->v.size());
     hoff += align256(sizeof(Region) * std::max<size_t>(1, lists[l]->v.size()));
   }
@@ -393,28 +414,30 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
   };
   const double decay_full = C.dedner ? std::exp(-dt * C.ch * C.ch / (C.cp * C.cp)) : 1.0;
   const double decay_half = C.dedner ? std::exp(-(0.5 * dt) * C.ch * C.ch / (C.cp * C.cp)) : 1.0;
+  auto stage = [&](int l0) {
+    StageLists L;
+    L.prim = rl(l0);
+    for (int d = 0; d < 3; ++d) {
+      L.flux[d] = rl(l0 + 1 + d);
+      L.fbc[d] = rl(l0 + 4 + d);
+      if (d >= dims) { L.flux[d].nblk = 0; L.fbc[d].nblk = 0; }
+    }
+    L.upd = rl(l0 + 7);
+    return L;
+  };
   if (rk2) {
-    StageLists L1;
-    L1.prim = rl(0); L1.flux[0] = rl(1); L1.flux[1] = rl(2); L1.flux[2] = rl(3); L1.upd = rl(4);
-    for (int d = dims; d < 3; ++d) L1.flux[d].nblk = 0;
     StepArgs S1{dt, dx, dt + time, decay_full, 0, 1, 0, 0, 0};
-    e = launch(L1, S1);
+    e = launch(stage(0), S1);
     if (e != hipSuccess) return hipfail(c, e, "stage 1 launch");
-    StageLists L2;
-    L2.prim = rl(5); L2.flux[0] = rl(6); L2.flux[1] = rl(7); L2.flux[2] = rl(8); L2.upd = rl(9);
-    for (int d = dims; d < 3; ++d) L2.flux[d].nblk = 0;
     StepArgs S2{0.5 * dt, dx, 0.5 * dt + time, decay_half, 1, 0, 1, 1, 1};
-    e = launch(L2, S2);
+    e = launch(stage(8), S2);
     if (e != hipSuccess) return hipfail(c, e, "stage 2 launch");
   } else {
-    StageLists L2;
-    L2.prim = rl(5); L2.flux[0] = rl(6); L2.flux[1] = rl(7); L2.flux[2] = rl(8); L2.upd = rl(9);
-    for (int d = dims; d < 3; ++d) L2.flux[d].nblk = 0;
     StepArgs S{dt, dx, dt + time, decay_full, 0, 0, 0, 2, 1};
-    e = launch(L2, S);
+    e = launch(stage(8), S);
     if (e != hipSuccess) return hipfail(c, e, "first-order launch");
   }
-  RegionList U2 = rl(9);
+  RegionList U2 = rl(15);
   e = launch_dt_final(U2, dtp, idp, reinterpret_cast<double*>(A + off_dto),
                       reinterpret_cast<long*>(A + off_ido), s);
   if (e != hipSuccess) return hipfail(c, e, "dt reduction launch");

@@ -42,8 +42,10 @@ struct RegionList {
   int nblk;         // total blocks
 };
 
+// flux[d]: interior faces of direction d; fbc[d]: faces of direction d on
+// a physical (non-periodic) domain boundary.
 struct StageLists {
-  RegionList prim, flux[3], upd;
+  RegionList prim, flux[3], fbc[3], upd;
 };
 
 struct StepArgs {
