@@ -25,6 +25,11 @@ import mfmhd  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 
+# HBM bytes per k_update launch (average over the two RK2 stages) measured
+# with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this workload, scaled by
+# the copy-kernel calibration; see profiles/k_update_traffic_256.txt.
+TRAFFIC = {}
+
 
 def c3_params(n, prec):
     dx = 1.0 / n
@@ -86,6 +91,9 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=40)
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device-to-device once "
+                         "(calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -113,6 +121,8 @@ def main():
         dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
         t += dt
     mfmhd.synchronize()
+    # per-kernel HIP events on the launch stream over the timed region
+    ctx.set_timing(True)
     e0, e1 = mfmhd.Event(), mfmhd.Event()
     e0.record()
     w0 = time.perf_counter()
@@ -126,8 +136,29 @@ def main():
     ms = e0.elapsed_ms(e1) / args.steps
     cells = float(n) ** 3
     value = cells / (ms * 1e-3)
-    alg_bytes = 5 * nc * 8  # per cell-update (RK2: read U, write U*, read U*, U, write U)
-    achieved = alg_bytes * value
+    fs = 4 if prec == mfmhd.PREC_FP32_FLUX else 8
+    kern = {}
+    for st in (0, 1):
+        for kind, name in enumerate(mfmhd.KT_NAMES):
+            kms, kn, kit = ctx.timing(st, kind)
+            if kn:
+                kern["%s/stage%d" % (name, st + 1)] = {
+                    "avg_ms": kms / kn, "launches": kn, "items_per_launch": kit / kn}
+    # dominant kernel: k_update (flux divergence, sources, floors, CFL).
+    # Algorithmic bytes per cell: read U_in (nc*8), read the face fluxes and
+    # normal B of three directions once (3*(nc+1)*fs), write U_out (nc*8);
+    # stage 1 also writes W of U* (nc*fs), stage 2 reads U^n (nc*8).
+    upd_bytes = {0: nc * 8 + 3 * (nc + 1) * fs + nc * 8 + nc * fs,
+                 1: nc * 8 + 3 * (nc + 1) * fs + nc * 8 + nc * 8}
+    tot_bytes = tot_ms = 0.0
+    launches = 0
+    for st in (0, 1):
+        kms, kn, kit = ctx.timing(st, mfmhd.KT_UPDATE)
+        tot_bytes += upd_bytes[st] * kit
+        tot_ms += kms
+        launches += kn
+    achieved = tot_bytes / (tot_ms * 1e-3)
+    step_bytes = 5 * nc * 8  # whole RK2 step: U in/out per stage + U^n re-read
     out = {
         "metric": "cell-updates/s per MF-MHD step",
         "value": value,
@@ -145,14 +176,27 @@ def main():
                                "neutrals (23 comps), RK2, Roe8, minmod, 8-wave, "
                                "charge exchange, single patch" % n,
                    "cells": int(cells), "components": nc, "precision": args.precision},
-        "roofline": {"bound": "hbm", "kernel": "whole RK2 patch update",
+        "roofline": {"bound": "hbm", "kernel": "k_update",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
-                     "alg_bytes_per_cell_update": alg_bytes},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": TRAFFIC.get((n, args.precision)),
+                     "alg_bytes_per_launch": tot_bytes / launches,
+                     "avg_launch_ms": tot_ms / launches,
+                     "alg_bytes_per_cell": {"stage1": upd_bytes[0], "stage2": upd_bytes[1]}},
+        "step_hbm_frac": step_bytes * value / HBM_PEAK,
+        "kernels": kern,
         "wall_s_timed": wall,
     }
     ctx.close()
     U.free()
+    if args.calib_copy > 0:
+        nb = args.calib_copy << 20
+        a = mfmhd.DeviceArray((nb // 8,), zero=True)
+        b = mfmhd.DeviceArray((nb // 8,))
+        b.copy_from(a)
+        mfmhd.synchronize()
+        a.free()
+        b.free()
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cb = cpu_baseline(args.cpu_sample, threads)

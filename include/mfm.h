@@ -167,6 +167,24 @@ int mfm_ctx_set_dedner(mfm_ctx* ctx, double ch, double cp);
 /* Last error message of this context (static storage, never NULL). */
 const char* mfm_ctx_last_error(const mfm_ctx* ctx);
 
+/* Kernel timing (instrumentation; no counterpart in the reference, which
+ * times whole steps with its own timers).  When on, HIP events bracket
+ * every kernel launch of the updates on their stream; get_timing returns,
+ * per RK2 stage (0 = predictor, 1 = corrector or the single first-order
+ * stage) and kernel kind, the summed milliseconds, the number of launches
+ * and the summed work items (cells, or faces for the flux kernels).
+ * set_timing resets the sums. */
+enum {
+  MFM_KT_PRIM = 0,         /* cons->prim and physical ghost states       */
+  MFM_KT_FLUX_PLASMA = 1,  /* interior faces, plasma Riemann flux        */
+  MFM_KT_FLUX_NEUTRAL = 2, /* interior faces, neutral van Leer fluxes    */
+  MFM_KT_FLUX_BC = 3,      /* faces on a physical boundary (fluxBC)      */
+  MFM_KT_UPDATE = 4        /* flux divergence, sources, floors, CFL      */
+};
+int mfm_ctx_set_timing(mfm_ctx* ctx, int on);
+int mfm_ctx_get_timing(mfm_ctx* ctx, int stage, int kind, double* ms,
+                       long* launches, double* items);
+
 /* ---- one patch: PatchMHDAM::updateState ---------------------------------
  * U      : device FAB, box = valid grown by >= numGhostCells, ghosts filled
  *          (exchange / interpolation) as the reference requires; updated in

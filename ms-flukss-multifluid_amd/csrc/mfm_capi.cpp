@@ -62,6 +62,11 @@ struct mfm_ctx {
   char* hbuf = nullptr;
   size_t hbuf_bytes = 0;
   size_t fsize() const { return p.precision == MFM_PREC_FP32_FLUX ? 4 : 8; }
+  // optional per-kernel timing, accumulated per stage and kernel kind
+  mfm::KernelTimer* timer = nullptr;
+  double kt_ms[2][mfm::KK_COUNT] = {};
+  long kt_n[2][mfm::KK_COUNT] = {};
+  double kt_items[2][mfm::KK_COUNT] = {};
 };
 
 namespace {
@@ -126,9 +131,11 @@ HBox clip(const Consts& C, HBox b) {
 struct RegionBuilder {
   std::vector<Region> v;
   int nblk = 0;
+  long items = 0;
   void add(int patch, const HBox& b) {
     long cells = b.size();
     if (cells <= 0) return;
+    items += cells;
     Region r;
     r.patch = patch;
     for (int d = 0; d < 3; ++d) { r.lo[d] = b.lo[d]; r.n[d] = (int)b.n(d); }
@@ -215,6 +222,9 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
   C.c13493 = p->ce_single_literals ? (double)13.493f : 13.493;
   C.c0531 = p->ce_single_literals ? (double)0.531f : 0.531;
   C.ce_on = (p->problem == MFM_PROBLEM_HELIO && p->nfluids >= 2) ? 1 : 0;
+  C.sin_tilt = std::sin(p->sun_tilt);
+  C.cos_tilt = std::cos(p->sun_tilt);
+  C.sun_beta = p->problem == MFM_PROBLEM_HELIO ? p->omega * kAU / (p->sun_v * p->lism_v_dim) : 0.0;
   return MFM_OK;
 }
 
@@ -255,6 +265,8 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
         return fail(c, MFM_ERR_SHAPE, "unused dimensions must be a single cell");
     if (!contains(ub, grow(v, ng, dims)))
       return fail(c, MFM_ERR_SHAPE, "U box must contain the valid box grown by numGhostCells");
+    if (ub.size() >= (1L << 31))
+      return fail(c, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^31 cells");
     for (int d = 0; d < dims; ++d)
       if (C0.phys[d] && (v.lo[d] < C0.dom_lo[d] || v.hi[d] > C0.dom_hi[d]))
         return fail(c, MFM_ERR_SHAPE, "valid box extends outside the problem domain");
@@ -277,6 +289,8 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
         want.hi[d] += 1;
         if (!P.F[d].data || P.F[d].ncomp != C0.nc || !contains(fo, want))
           return fail(c, MFM_ERR_SHAPE, "F[d] must cover the faces of the valid box with ncomp components");
+        if (fo.size() >= (1L << 31))
+          return fail(c, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^31 cells");
       }
     }
     if (P.divB && (!P.divB->data || P.divB->ncomp < 1))
@@ -309,7 +323,27 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
       for (int d = 0; d < dims; ++d) add_faces(st1_flux[d], st1_fbc[d], q, g.fb[d], d);
       st1_upd.add(q, g.vb1);
     }
-    st2_prim.add(q, grow(g.vb2, 2, dims));
+    if (rk2) {
+      // the first stage leaves W of U* on vb1; the second stage only needs
+      // the physical ghost cells of grow(vb2, 2) outside it
+      HBox rest = grow(g.vb2, 2, dims);
+      for (int d = 0; d < dims; ++d) {
+        if (rest.lo[d] < g.vb1.lo[d]) {
+          HBox b = rest;
+          b.hi[d] = g.vb1.lo[d] - 1;
+          st2_prim.add(q, b);
+          rest.lo[d] = g.vb1.lo[d];
+        }
+        if (rest.hi[d] > g.vb1.hi[d]) {
+          HBox b = rest;
+          b.lo[d] = g.vb1.hi[d] + 1;
+          st2_prim.add(q, b);
+          rest.hi[d] = g.vb1.hi[d];
+        }
+      }
+    } else {
+      st2_prim.add(q, grow(g.vb2, 2, dims));
+    }
     for (int d = 0; d < dims; ++d) {
       HBox f = g.vb2;
       f.hi[d] += 1;
@@ -401,6 +435,7 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
     r.r = reinterpret_cast<const Region*>(A + off_reg[l]);
     r.n = (int)lists[l]->v.size();
     r.nblk = lists[l]->nblk;
+    r.items = lists[l]->items;
     return r;
   };
   const DevPatch* dp = reinterpret_cast<const DevPatch*>(A + off_patch);
@@ -409,8 +444,8 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
   Consts C = C0;
   if (p.problem == MFM_PROBLEM_HELIO) C.r0_dt_ignore = p.sun_r0 - 2.0 * dx;  // define()
   auto launch = [&](const StageLists& L, const StepArgs& S) -> hipError_t {
-    if (p.precision == MFM_PREC_FP32_FLUX) return launch_stage<float>(C, dp, L, S, dtp, idp, s);
-    return launch_stage<double>(C, dp, L, S, dtp, idp, s);
+    if (p.precision == MFM_PREC_FP32_FLUX) return launch_stage<float>(C, dp, L, S, dtp, idp, s, c->timer);
+    return launch_stage<double>(C, dp, L, S, dtp, idp, s, c->timer);
   };
   const double decay_full = C.dedner ? std::exp(-dt * C.ch * C.ch / (C.cp * C.cp)) : 1.0;
   const double decay_half = C.dedner ? std::exp(-(0.5 * dt) * C.ch * C.ch / (C.cp * C.cp)) : 1.0;
@@ -425,15 +460,18 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
     L.upd = rl(l0 + 7);
     return L;
   };
+  if (c->timer) c->timer->n = 0;
+  int n_stage1 = 0;
   if (rk2) {
-    StepArgs S1{dt, dx, dt + time, decay_full, 0, 1, 0, 0, 0};
+    StepArgs S1{dt, dx, dt + time, decay_full, 0, 1, 0, 0, 0, 1};
     e = launch(stage(0), S1);
     if (e != hipSuccess) return hipfail(c, e, "stage 1 launch");
-    StepArgs S2{0.5 * dt, dx, 0.5 * dt + time, decay_half, 1, 0, 1, 1, 1};
+    if (c->timer) n_stage1 = c->timer->n;
+    StepArgs S2{0.5 * dt, dx, 0.5 * dt + time, decay_half, 1, 0, 1, 1, 1, 0};
     e = launch(stage(8), S2);
     if (e != hipSuccess) return hipfail(c, e, "stage 2 launch");
   } else {
-    StepArgs S{dt, dx, dt + time, decay_full, 0, 0, 0, 2, 1};
+    StepArgs S{dt, dx, dt + time, decay_full, 0, 0, 0, 2, 1, 0};
     e = launch(stage(8), S);
     if (e != hipSuccess) return hipfail(c, e, "first-order launch");
   }
@@ -447,6 +485,18 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
   if (e == hipSuccess) e = hipMemcpyAsync(hid, A + off_ido, sizeof(long) * n, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hipfail(c, e, "patch update");
+  if (c->timer) {
+    mfm::KernelTimer& T = *c->timer;
+    for (int q = 0; q < T.n; ++q) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, T.ev[2 * q], T.ev[2 * q + 1]) != hipSuccess) continue;
+      const int st = (rk2 && q < n_stage1) ? 0 : 1;
+      c->kt_ms[st][T.kind[q]] += ms;
+      c->kt_n[st][T.kind[q]] += 1;
+      c->kt_items[st][T.kind[q]] += (double)T.items[q];
+    }
+    T.n = 0;
+  }
   // per-patch results; regions of the last update list are one per patch
   double best = std::numeric_limits<double>::max();
   int bestp = -1;
@@ -647,9 +697,47 @@ int mfm_ctx_create(const mfm_params* p, mfm_ctx** out) {
   return MFM_OK;
 }
 
+int mfm_ctx_set_timing(mfm_ctx* c, int on) {
+  if (!c) return MFM_ERR_ARG;
+  if (on && !c->timer) {
+    mfm::KernelTimer* T = new mfm::KernelTimer();
+    for (int q = 0; q < 2 * mfm::KernelTimer::kMax; ++q) {
+      hipError_t e = hipEventCreate(&T->ev[q]);
+      if (e != hipSuccess) {
+        for (int r = 0; r < q; ++r) (void)hipEventDestroy(T->ev[r]);
+        delete T;
+        return hipfail(c, e, "hipEventCreate");
+      }
+    }
+    c->timer = T;
+  } else if (!on && c->timer) {
+    (void)hipDeviceSynchronize();
+    for (int q = 0; q < 2 * mfm::KernelTimer::kMax; ++q) (void)hipEventDestroy(c->timer->ev[q]);
+    delete c->timer;
+    c->timer = nullptr;
+  }
+  for (int st = 0; st < 2; ++st)
+    for (int k = 0; k < mfm::KK_COUNT; ++k) {
+      c->kt_ms[st][k] = 0.0;
+      c->kt_n[st][k] = 0;
+      c->kt_items[st][k] = 0.0;
+    }
+  return MFM_OK;
+}
+
+int mfm_ctx_get_timing(mfm_ctx* c, int stage, int kind, double* ms, long* launches,
+                       double* items) {
+  if (!c || stage < 0 || stage > 1 || kind < 0 || kind >= mfm::KK_COUNT) return MFM_ERR_ARG;
+  if (ms) *ms = c->kt_ms[stage][kind];
+  if (launches) *launches = c->kt_n[stage][kind];
+  if (items) *items = c->kt_items[stage][kind];
+  return MFM_OK;
+}
+
 int mfm_ctx_destroy(mfm_ctx* c) {
   if (!c) return MFM_OK;
   (void)hipDeviceSynchronize();
+  if (c->timer) (void)mfm_ctx_set_timing(c, 0);
   if (c->arena) (void)hipFree(c->arena);
   if (c->hbuf) (void)hipHostFree(c->hbuf);
   delete c;

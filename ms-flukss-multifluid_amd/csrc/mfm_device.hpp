@@ -61,6 +61,8 @@ struct Consts {
       r0_dt_ignore;
   double ce_scale_vel, ce_sigma, c13493, c0531;
   int ce_on;
+  // uniform factors of HELIOREINIT_DEFAULT, formed once on the host
+  double sin_tilt, cos_tilt, sun_beta;
 };
 
 template <typename T>

@@ -40,6 +40,7 @@ struct RegionList {
   const Region* r;  // device pointer
   int n;            // number of regions
   int nblk;         // total blocks
+  long items;       // cells (or faces) covered
 };
 
 // flux[d]: interior faces of direction d; fbc[d]: faces of direction d on
@@ -58,12 +59,35 @@ struct StepArgs {
   int average;         // RK2 second stage: U += (U^n - U*)/2
   int fo_mode;         // caller fluxes: 0 none, 1 average with stage 1, 2 copy
   int reduce_dt;       // compute the CFL reduction from the stage output
+  int write_w;         // store W of the stage output (input of the next stage)
+};
+
+// Optional kernel timing: a pair of events brackets every launch on the
+// launch stream; the host reads them after the step has synchronized.
+enum KernelKind { KK_PRIM = 0, KK_FLUX_PLASMA, KK_FLUX_NEUTRAL, KK_FLUX_BC, KK_UPDATE, KK_COUNT };
+struct KernelTimer {
+  static constexpr int kMax = 32;
+  hipEvent_t ev[2 * kMax];
+  int kind[kMax];
+  long items[kMax];
+  int n = 0;
+  void begin(int k, long it, hipStream_t s) {
+    if (n >= kMax) return;
+    kind[n] = k;
+    items[n] = it;
+    (void)hipEventRecord(ev[2 * n], s);
+  }
+  void end(hipStream_t s) {
+    if (n >= kMax) return;
+    (void)hipEventRecord(ev[2 * n + 1], s);
+    ++n;
+  }
 };
 
 template <typename TF>
 hipError_t launch_stage(const Consts& C, const DevPatch* P, const StageLists& L,
                         const StepArgs& A, double* dt_part, long* idx_part,
-                        hipStream_t s);
+                        hipStream_t s, KernelTimer* T);
 
 hipError_t launch_dt_final(const RegionList& L, const double* dt_part,
                            const long* idx_part, double* dt_out, long* idx_out,

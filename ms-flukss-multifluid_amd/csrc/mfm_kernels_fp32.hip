@@ -7,5 +7,5 @@ namespace mfm {
 template hipError_t launch_stage<float>(const Consts&, const DevPatch*,
                                         const StageLists&,
                                         const StepArgs&, double*, long*,
-                                        hipStream_t);
+                                        hipStream_t, KernelTimer*);
 }  // namespace mfm

@@ -8,7 +8,7 @@ namespace mfm {
 template hipError_t launch_stage<double>(const Consts&, const DevPatch*,
                                          const StageLists&,
                                          const StepArgs&, double*, long*,
-                                         hipStream_t);
+                                         hipStream_t, KernelTimer*);
 
 hipError_t launch_dt_final(const RegionList& L, const double* dt_part,
                            const long* idx_part, double* dt_out, long* idx_out,

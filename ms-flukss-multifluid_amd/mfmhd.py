@@ -95,7 +95,12 @@ EXPORTS = [
     "mfm_dev_count", "mfm_dev_alloc", "mfm_dev_free", "mfm_dev_memcpy",
     "mfm_dev_memset", "mfm_dev_synchronize", "mfm_event_create",
     "mfm_event_destroy", "mfm_event_record", "mfm_event_elapsed_ms",
+    "mfm_ctx_set_timing", "mfm_ctx_get_timing",
 ]
+
+# kernel kinds of mfm_ctx_get_timing
+KT_PRIM, KT_FLUX_PLASMA, KT_FLUX_NEUTRAL, KT_FLUX_BC, KT_UPDATE = range(5)
+KT_NAMES = ("k_prim", "k_flux_plasma", "k_flux_neutral", "k_flux_bc", "k_update")
 
 _lib = None
 
@@ -128,6 +133,11 @@ def load(path=LIB_PATH):
     lib.mfm_ctx_destroy.argtypes = [C.c_void_p]
     lib.mfm_ctx_set_dedner.argtypes = [C.c_void_p, C.c_double, C.c_double]
     lib.mfm_ctx_last_error.argtypes = [C.c_void_p]
+    lib.mfm_ctx_set_timing.argtypes = [C.c_void_p, C.c_int]
+    lib.mfm_ctx_get_timing.argtypes = [C.c_void_p, C.c_int, C.c_int,
+                                       C.POINTER(C.c_double),
+                                       C.POINTER(C.c_long),
+                                       C.POINTER(C.c_double)]
     lib.mfm_ctx_last_error.restype = C.c_char_p
     lib.mfm_patch_update_dev.argtypes = [
         C.c_void_p, Fab, Box, C.c_double, C.c_double, C.c_double,
@@ -257,6 +267,18 @@ class Context:
     def set_dedner(self, ch, cp):
         self._check(self.lib.mfm_ctx_set_dedner(self.h, ch, cp))
 
+    def set_timing(self, on):
+        """Bracket every kernel launch with HIP events (resets the sums)."""
+        self._check(self.lib.mfm_ctx_set_timing(self.h, 1 if on else 0))
+
+    def timing(self, stage, kind):
+        """(milliseconds, launches, work items) summed for one RK2 stage
+        (0 predictor, 1 corrector / single stage) and kernel kind."""
+        ms, n, it = C.c_double(), C.c_long(), C.c_double()
+        self._check(self.lib.mfm_ctx_get_timing(self.h, stage, kind, C.byref(ms),
+                                                C.byref(n), C.byref(it)))
+        return ms.value, n.value, it.value
+
     def update_state(self, U, valid, dt, dx, time=0.0, F=None, divB=None,
                      stream=None, check=True):
         """PatchMHDAM::updateState on a device FAB; returns (new_dt, cell)."""
@@ -349,6 +371,14 @@ class DeviceArray:
                                      self.nbytes, 0)
         if rc != OK:
             raise MfmError(rc, "host->device copy failed")
+
+    def copy_from(self, other):
+        """Device-to-device copy of an array of the same size."""
+        assert other.nbytes == self.nbytes
+        rc = self.lib.mfm_dev_memcpy(C.c_void_p(self.ptr), C.c_void_p(other.ptr),
+                                     self.nbytes, 2)
+        if rc != OK:
+            raise MfmError(rc, "device->device copy failed")
 
     def numpy(self):
         import numpy as np
