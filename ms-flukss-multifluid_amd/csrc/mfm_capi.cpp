@@ -215,7 +215,6 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
   C.lism_bx = p->lism_bx; C.lism_by = p->lism_by; C.lism_bz = p->lism_bz;
   C.net_rho = p->net_rho; C.net_p = p->net_p; C.tmlim = p->tmlim;
   C.reg_safe_z_top = p->reg_safe_z_top; C.reg_safe_z_bot = p->reg_safe_z_bot;
-  C.r0_dt_ignore = -1.0;
   C.ce_scale_vel = p->ce_scale_vel;
   C.ce_sigma = p->ce_sigma_lindsay;
   // Fortran default-real literals of the cross section (see include/mfm.h)
@@ -233,6 +232,8 @@ struct PatchIn {
   mfm_box valid;
   mfm_fab* F;
   mfm_fab* divB;
+  int cidx;                // row of the constants table
+  double dt, dx, time;
 };
 
 // Geometry of one patch's scratch.
@@ -241,21 +242,30 @@ struct PatchGeo {
   size_t off_up, off_w, off_f[3], off_bn[3];
 };
 
-int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
-               double time, double* new_dt, int* min_cell, double* level_dt,
-               int* min_patch, hipStream_t s) {
-  const Consts& C0 = c->C;
+// One batched update of n patches.  Each patch names a row of the
+// constants table Ct (nct rows) and carries its own dt, dx and time; all
+// rows share the launch class of c (dimension, fluids, GLM, time scheme,
+// precision).  c provides the scratch arena, staging, timer and errors.
+int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
+              double* new_dt, int* min_cell, double* level_dt, int* min_patch,
+              hipStream_t s) {
+  const Consts& C0 = Ct[0];
   const mfm_params& p = c->p;
   const int dims = C0.dims;
   const int ng = p.time_approx == MFM_TIME_RK2 ? 4 : 2;
   const bool rk2 = p.time_approx == MFM_TIME_RK2;
   if (n <= 0) return fail(c, MFM_ERR_ARG, "empty patch table");
-  if (!(dt > 0.0) || !(dx > 0.0)) return fail(c, MFM_ERR_ARG, "dt and dx must be positive");
+  for (int r = 1; r < nct; ++r)
+    if (Ct[r].dims != C0.dims || Ct[r].nf != C0.nf || Ct[r].dedner != C0.dedner)
+      return fail(c, MFM_ERR_ARG, "constants of one batch must share dimension, fluids and GLM");
   std::vector<PatchGeo> G(n);
   size_t off = 0;
   const size_t fs = c->fsize();
   for (int q = 0; q < n; ++q) {
     const PatchIn& P = in[q];
+    if (P.cidx < 0 || P.cidx >= nct) return fail(c, MFM_ERR_ARG, "bad constants row");
+    const Consts& Cq = Ct[P.cidx];
+    if (!(P.dt > 0.0) || !(P.dx > 0.0)) return fail(c, MFM_ERR_ARG, "dt and dx must be positive");
     if (!P.U.data) return fail(c, MFM_ERR_ARG, "U.data is NULL");
     if (P.U.ncomp != C0.nc) return fail(c, MFM_ERR_SHAPE, "U.ncomp does not match the equation system");
     HBox ub = hb(P.U.box), v = hb(P.valid);
@@ -268,12 +278,12 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
     if (ub.size() >= (1L << 31))
       return fail(c, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^31 cells");
     for (int d = 0; d < dims; ++d)
-      if (C0.phys[d] && (v.lo[d] < C0.dom_lo[d] || v.hi[d] > C0.dom_hi[d]))
+      if (Cq.phys[d] && (v.lo[d] < Cq.dom_lo[d] || v.hi[d] > Cq.dom_hi[d]))
         return fail(c, MFM_ERR_SHAPE, "valid box extends outside the problem domain");
     PatchGeo& g = G[q];
     g.ub = ub;
-    g.vb2 = clip(C0, v);
-    g.vb1 = rk2 ? clip(C0, grow(v, 2, dims)) : g.vb2;
+    g.vb2 = clip(Cq, v);
+    g.vb1 = rk2 ? clip(Cq, grow(v, 2, dims)) : g.vb2;
     const long ucells = ub.size();
     g.off_up = off; off += align256((size_t)ucells * C0.nc * 8);
     g.off_w = off; off += align256((size_t)ucells * C0.nc * fs);
@@ -297,17 +307,19 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
       return fail(c, MFM_ERR_SHAPE, "divB must have one component");
   }
   // region lists; faces on a physical boundary go to their own list
+  const PatchIn* pin = in;
   RegionBuilder st1_prim, st1_flux[3], st1_fbc[3], st1_upd, st2_prim, st2_flux[3],
       st2_fbc[3], st2_upd;
   auto add_faces = [&](RegionBuilder& in, RegionBuilder& bc, int q, HBox f, int d) {
-    if (C0.phys[d]) {
-      if (f.lo[d] == C0.dom_lo[d]) {
+    const Consts& Cq = Ct[pin[q].cidx];
+    if (Cq.phys[d]) {
+      if (f.lo[d] == Cq.dom_lo[d]) {
         HBox b = f;
         b.hi[d] = b.lo[d];
         bc.add(q, b);
         f.lo[d] += 1;
       }
-      if (f.hi[d] == C0.dom_hi[d] + 1 && f.hi[d] >= f.lo[d]) {
+      if (f.hi[d] == Cq.dom_hi[d] + 1 && f.hi[d] >= f.lo[d]) {
         HBox b = f;
         b.lo[d] = b.hi[d];
         bc.add(q, b);
@@ -360,12 +372,17 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
   size_t off_patch = off; off += align256(sizeof(DevPatch) * n);
   size_t off_reg[kLists];
   size_t table_bytes = align256(sizeof(DevPatch) * n);
+  // constants table after the region lists
+
   for (int l = 0; l < kLists; ++l) {
     off_reg[l] = off;
     size_t b = align256(sizeof(Region) * std::max<size_t>(1, lists[l]->v.size()));
     off += b;
     table_bytes += b;
   }
+  size_t off_ct = off;
+  off += align256(sizeof(Consts) * nct);
+  table_bytes += align256(sizeof(Consts) * nct);
   const int nbu = std::max(1, st2_upd.nblk);
   size_t off_dtp = off; off += align256(sizeof(double) * nbu);
   size_t off_idp = off; off += align256(sizeof(long) * nbu);
@@ -380,12 +397,19 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
   // host staging of the tables
   char* h = c->hbuf;
   DevPatch* hp = reinterpret_cast<DevPatch*>(h);
-  const double sunR = p.sun_r0;
   for (int q = 0; q < n; ++q) {
     const PatchIn& P = in[q];
+    const Consts& Cq = Ct[P.cidx];
     const PatchGeo& g = G[q];
     DevPatch& D = hp[q];
     std::memset(&D, 0, sizeof(D));
+    D.cidx = P.cidx;
+    D.dx = P.dx;
+    D.dt = P.dt;
+    D.time = P.time;
+    // PSIPARABOLIC factor exp(-dt_stage Ch^2/Cp^2) of each stage
+    D.psi_decay[0] = Cq.dedner ? std::exp(-P.dt * Cq.ch * Cq.ch / (Cq.cp * Cq.cp)) : 1.0;
+    D.psi_decay[1] = Cq.dedner ? std::exp(-(0.5 * P.dt) * Cq.ch * Cq.ch / (Cq.cp * Cq.cp)) : 1.0;
     D.U = P.U.data;
     D.Up = reinterpret_cast<double*>(A + g.off_up);
     D.W = A + g.off_w;
@@ -413,10 +437,11 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
     }
     // HELIOREINIT_DEFAULT's early exit on the FAB bounds
     D.reinit = 0;
-    if (p.problem == MFM_PROBLEM_HELIO) {
-      double xl = g.ub.lo[0] * dx - p.sun_xc, yl = g.ub.lo[1] * dx - p.sun_yc, zl = g.ub.lo[2] * dx - p.sun_zc;
-      double xh = (g.ub.hi[0] + 1) * dx - p.sun_xc, yh = (g.ub.hi[1] + 1) * dx - p.sun_yc,
-             zh = (g.ub.hi[2] + 1) * dx - p.sun_zc;
+    if (Cq.problem == MFM_PROBLEM_HELIO) {
+      const double dx = P.dx, sunR = Cq.sun_r0;
+      double xl = g.ub.lo[0] * dx - Cq.sun_xc, yl = g.ub.lo[1] * dx - Cq.sun_yc, zl = g.ub.lo[2] * dx - Cq.sun_zc;
+      double xh = (g.ub.hi[0] + 1) * dx - Cq.sun_xc, yh = (g.ub.hi[1] + 1) * dx - Cq.sun_yc,
+             zh = (g.ub.hi[2] + 1) * dx - Cq.sun_zc;
       bool out = (xl > sunR) || (yl > sunR) || (dims == 3 && zl > sunR) || (xh < -sunR) ||
                  (yh < -sunR) || (dims == 3 && zh < -sunR);
       D.reinit = out ? 0 : 1;
@@ -427,6 +452,7 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
     std::memcpy(h + hoff, lists[l]->v.data(), sizeof(Region) * lists[l]->v.size());
     hoff += align256(sizeof(Region) * std::max<size_t>(1, lists[l]->v.size()));
   }
+  std::memcpy(h + hoff, Ct, sizeof(Consts) * nct);
   hipError_t e = hipMemcpyAsync(A + off_patch, h, table_bytes, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hipfail(c, e, "hipMemcpyAsync(tables)");
 
@@ -441,14 +467,11 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
   const DevPatch* dp = reinterpret_cast<const DevPatch*>(A + off_patch);
   double* dtp = reinterpret_cast<double*>(A + off_dtp);
   long* idp = reinterpret_cast<long*>(A + off_idp);
-  Consts C = C0;
-  if (p.problem == MFM_PROBLEM_HELIO) C.r0_dt_ignore = p.sun_r0 - 2.0 * dx;  // define()
+  const Consts* ct = reinterpret_cast<const Consts*>(A + off_ct);
   auto launch = [&](const StageLists& L, const StepArgs& S) -> hipError_t {
-    if (p.precision == MFM_PREC_FP32_FLUX) return launch_stage<float>(C, dp, L, S, dtp, idp, s, c->timer);
-    return launch_stage<double>(C, dp, L, S, dtp, idp, s, c->timer);
+    if (p.precision == MFM_PREC_FP32_FLUX) return launch_stage<float>(C0, ct, dp, L, S, dtp, idp, s, c->timer);
+    return launch_stage<double>(C0, ct, dp, L, S, dtp, idp, s, c->timer);
   };
-  const double decay_full = C.dedner ? std::exp(-dt * C.ch * C.ch / (C.cp * C.cp)) : 1.0;
-  const double decay_half = C.dedner ? std::exp(-(0.5 * dt) * C.ch * C.ch / (C.cp * C.cp)) : 1.0;
   auto stage = [&](int l0) {
     StageLists L;
     L.prim = rl(l0);
@@ -463,15 +486,15 @@ int run_update(mfm_ctx* c, const PatchIn* in, int n, double dt, double dx,
   if (c->timer) c->timer->n = 0;
   int n_stage1 = 0;
   if (rk2) {
-    StepArgs S1{dt, dx, dt + time, decay_full, 0, 1, 0, 0, 0, 1};
+    StepArgs S1{1.0, 0, 0, 1, 0, 0, 0, 1};
     e = launch(stage(0), S1);
     if (e != hipSuccess) return hipfail(c, e, "stage 1 launch");
     if (c->timer) n_stage1 = c->timer->n;
-    StepArgs S2{0.5 * dt, dx, 0.5 * dt + time, decay_half, 1, 0, 1, 1, 1, 0};
+    StepArgs S2{0.5, 1, 1, 0, 1, 1, 1, 0};
     e = launch(stage(8), S2);
     if (e != hipSuccess) return hipfail(c, e, "stage 2 launch");
   } else {
-    StepArgs S{dt, dx, dt + time, decay_full, 0, 0, 0, 2, 1, 0};
+    StepArgs S{1.0, 0, 0, 0, 0, 2, 1, 0};
     e = launch(stage(8), S);
     if (e != hipSuccess) return hipfail(c, e, "first-order launch");
   }
@@ -763,11 +786,10 @@ int mfm_patch_update_dev(mfm_ctx* c, mfm_fab U, mfm_box valid, double dt,
                          double dx, double time, mfm_fab* F, mfm_fab* divB,
                          double* new_dt, int min_cell[3], void* stream) {
   if (!c) return MFM_ERR_ARG;
-  PatchIn in{U, valid, F, divB};
+  PatchIn in{U, valid, F, divB, 0, dt, dx, time};
   double d;
   int cell[3];
-  int rc = run_update(c, &in, 1, dt, dx, time, &d, cell, nullptr, nullptr,
-                      (hipStream_t)stream);
+  int rc = run_batch(c, &c->C, 1, &in, 1, &d, cell, nullptr, nullptr, (hipStream_t)stream);
   if (new_dt) *new_dt = d;
   if (min_cell) { min_cell[0] = cell[0]; min_cell[1] = cell[1]; min_cell[2] = cell[2]; }
   return rc;
@@ -778,9 +800,10 @@ int mfm_level_update_dev(mfm_ctx* c, const mfm_patch* table, int n, double dt,
                          double* level_dt, int* min_patch, void* stream) {
   if (!c || !table) return MFM_ERR_ARG;
   std::vector<PatchIn> in(n);
-  for (int q = 0; q < n; ++q) in[q] = PatchIn{table[q].U, table[q].valid, table[q].F, table[q].divB};
-  return run_update(c, in.data(), n, dt, dx, time, new_dt, min_cell, level_dt,
-                    min_patch, (hipStream_t)stream);
+  for (int q = 0; q < n; ++q)
+    in[q] = PatchIn{table[q].U, table[q].valid, table[q].F, table[q].divB, 0, dt, dx, time};
+  return run_batch(c, &c->C, 1, in.data(), n, new_dt, min_cell, level_dt, min_patch,
+                   (hipStream_t)stream);
 }
 
 int mfm_patch_update_host(mfm_ctx* c, mfm_fab U, mfm_box valid, double dt,
@@ -846,17 +869,21 @@ int mfm_patch_compute_dt_dev(mfm_ctx* c, mfm_fab U, mfm_box box, double dx,
   if (U.ncomp != C0.nc) return fail(c, MFM_ERR_SHAPE, "U.ncomp mismatch");
   HBox ub = hb(U.box), b = hb(box);
   if (!contains(ub, b)) return fail(c, MFM_ERR_SHAPE, "box outside U");
+  if (ub.size() >= (1L << 31)) return fail(c, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^31 cells");
+  if (!(dx > 0.0)) return fail(c, MFM_ERR_ARG, "dx must be positive");
   RegionBuilder rb;
   rb.add(0, b);
   size_t off = 0;
   size_t off_p = off; off += align256(sizeof(DevPatch));
   size_t off_r = off; off += align256(sizeof(Region));
+  size_t off_c = off; off += align256(sizeof(Consts));
   size_t off_dtp = off; off += align256(sizeof(double) * rb.nblk);
   size_t off_idp = off; off += align256(sizeof(long) * rb.nblk);
   size_t off_dto = off; off += align256(sizeof(double));
   size_t off_ido = off; off += align256(sizeof(long));
   int rc = ensure_arena(c, off);
-  if (!rc) rc = ensure_hbuf(c, 4096);
+  if (!rc) rc = ensure_hbuf(c, 4096 + align256(sizeof(DevPatch)) + align256(sizeof(Region)) +
+                                 align256(sizeof(Consts)));
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   DevPatch* hp = reinterpret_cast<DevPatch*>(c->hbuf);
@@ -864,22 +891,23 @@ int mfm_patch_compute_dt_dev(mfm_ctx* c, mfm_fab U, mfm_box box, double dx,
   hp->U = U.data;
   for (int d = 0; d < 3; ++d) { hp->ulo[d] = ub.lo[d]; hp->un[d] = (int)ub.n(d); }
   hp->ucs = ub.size();
-  Region* hr = reinterpret_cast<Region*>(c->hbuf + align256(sizeof(DevPatch)));
+  hp->cidx = 0;
+  hp->dx = dx;
+  Region* hr = reinterpret_cast<Region*>(c->hbuf + off_r);
   *hr = rb.v[0];
+  std::memcpy(c->hbuf + off_c, &C0, sizeof(Consts));
   char* A = c->arena;
-  hipError_t e = hipMemcpyAsync(A + off_p, c->hbuf, align256(sizeof(DevPatch)) + sizeof(Region),
-                                hipMemcpyHostToDevice, s);
-  Consts C = C0;
-  if (c->p.problem == MFM_PROBLEM_HELIO) C.r0_dt_ignore = c->p.sun_r0 - 2.0 * dx;
-  RegionList L{reinterpret_cast<const Region*>(A + off_r), 1, rb.nblk};
+  hipError_t e = hipMemcpyAsync(A + off_p, c->hbuf, off_c + sizeof(Consts), hipMemcpyHostToDevice, s);
+  RegionList L{reinterpret_cast<const Region*>(A + off_r), 1, rb.nblk, rb.items};
   if (e == hipSuccess)
-    e = launch_dt_only(C, reinterpret_cast<const DevPatch*>(A + off_p), L, dx,
+    e = launch_dt_only(C0, reinterpret_cast<const Consts*>(A + off_c),
+                       reinterpret_cast<const DevPatch*>(A + off_p), L,
                        reinterpret_cast<double*>(A + off_dtp), reinterpret_cast<long*>(A + off_idp), s);
   if (e == hipSuccess)
     e = launch_dt_final(L, reinterpret_cast<double*>(A + off_dtp), reinterpret_cast<long*>(A + off_idp),
                         reinterpret_cast<double*>(A + off_dto), reinterpret_cast<long*>(A + off_ido), s);
-  double* hdt = reinterpret_cast<double*>(c->hbuf + 1024);
-  long* hid = reinterpret_cast<long*>(c->hbuf + 2048);
+  double* hdt = reinterpret_cast<double*>(c->hbuf + off_c + align256(sizeof(Consts)));
+  long* hid = reinterpret_cast<long*>(c->hbuf + off_c + align256(sizeof(Consts)) + 256);
   if (e == hipSuccess) e = hipMemcpyAsync(hdt, A + off_dto, sizeof(double), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(hid, A + off_ido, sizeof(long), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);

@@ -57,8 +57,7 @@ struct Consts {
       sun_tilt, sun_int_bc_radius, dir_br_n, omega, lism_v_dim, init_r;
   int sun_b_monopolar;
   double lism_ux, lism_uy, lism_uz, lism_p, lism_m, lism_t, lism_bx, lism_by,
-      lism_bz, net_rho, net_p, tmlim, reg_safe_z_top, reg_safe_z_bot,
-      r0_dt_ignore;
+      lism_bz, net_rho, net_p, tmlim, reg_safe_z_top, reg_safe_z_bot;
   double ce_scale_vel, ce_sigma, c13493, c0531;
   int ce_on;
   // uniform factors of HELIOREINIT_DEFAULT, formed once on the host

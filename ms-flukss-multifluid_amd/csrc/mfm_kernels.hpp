@@ -26,6 +26,9 @@ struct DevPatch {
   long focs[3];
   int dlo[3], dn[3];
   int reinit;       // patch box may intersect the heliospheric inner sphere
+  int cidx;         // row of the constants table
+  double dx, dt, time;   // cell size, step and time of this patch
+  double psi_decay[2];   // exp(-dt_stage Ch^2/Cp^2) of stage 0 / 1
 };
 
 // A box of work items belonging to one patch; blk0 = first block.
@@ -50,10 +53,8 @@ struct StageLists {
 };
 
 struct StepArgs {
-  double dt_stage;     // dt of this Euler stage
-  double dx;
-  double time_reinit;  // dt_stage + current time (problem postprocessing)
-  double psi_decay;    // exp(-dt Ch^2 / Cp^2)
+  double dt_frac;      // this Euler stage advances by dt_frac * patch dt
+  int stage;           // 0: first (or only) stage, 1: RK2 corrector
   int in_up;           // stage input: 0 = U, 1 = Up
   int out_up;          // stage output: 0 = U, 1 = Up
   int average;         // RK2 second stage: U += (U^n - U*)/2
@@ -85,15 +86,15 @@ struct KernelTimer {
 };
 
 template <typename TF>
-hipError_t launch_stage(const Consts& C, const DevPatch* P, const StageLists& L,
-                        const StepArgs& A, double* dt_part, long* idx_part,
-                        hipStream_t s, KernelTimer* T);
+hipError_t launch_stage(const Consts& C, const Consts* CT, const DevPatch* P,
+                        const StageLists& L, const StepArgs& A, double* dt_part,
+                        long* idx_part, hipStream_t s, KernelTimer* T);
 
 hipError_t launch_dt_final(const RegionList& L, const double* dt_part,
                            const long* idx_part, double* dt_out, long* idx_out,
                            hipStream_t s);
-hipError_t launch_dt_only(const Consts& C, const DevPatch* P, const RegionList& L,
-                          double dx, double* dt_part, long* idx_part,
+hipError_t launch_dt_only(const Consts& C, const Consts* CT, const DevPatch* P,
+                          const RegionList& L, double* dt_part, long* idx_part,
                           hipStream_t s);
 hipError_t launch_init(const Consts& C, double* U, const int ulo[3],
                        const int un[3], long ucs, const int lo[3],

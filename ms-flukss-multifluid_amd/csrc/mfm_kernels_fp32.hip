@@ -4,8 +4,7 @@
 #include "mfm_kernels.inc"
 
 namespace mfm {
-template hipError_t launch_stage<float>(const Consts&, const DevPatch*,
-                                        const StageLists&,
-                                        const StepArgs&, double*, long*,
-                                        hipStream_t, KernelTimer*);
+template hipError_t launch_stage<float>(const Consts&, const Consts*, const DevPatch*,
+                                         const StageLists&, const StepArgs&, double*,
+                                         long*, hipStream_t, KernelTimer*);
 }  // namespace mfm

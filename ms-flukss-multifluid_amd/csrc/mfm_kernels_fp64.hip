@@ -5,10 +5,9 @@
 #include "mfm_kernels.inc"
 
 namespace mfm {
-template hipError_t launch_stage<double>(const Consts&, const DevPatch*,
-                                         const StageLists&,
-                                         const StepArgs&, double*, long*,
-                                         hipStream_t, KernelTimer*);
+template hipError_t launch_stage<double>(const Consts&, const Consts*, const DevPatch*,
+                                         const StageLists&, const StepArgs&, double*,
+                                         long*, hipStream_t, KernelTimer*);
 
 hipError_t launch_dt_final(const RegionList& L, const double* dt_part,
                            const long* idx_part, double* dt_out, long* idx_out,
@@ -17,12 +16,12 @@ hipError_t launch_dt_final(const RegionList& L, const double* dt_part,
                      L.nblk, dt_part, idx_part, dt_out, idx_out);
   return hipGetLastError();
 }
-hipError_t launch_dt_only(const Consts& C, const DevPatch* P, const RegionList& L,
-                          double dx, double* dt_part, long* idx_part,
+hipError_t launch_dt_only(const Consts& C, const Consts* CT, const DevPatch* P,
+                          const RegionList& L, double* dt_part, long* idx_part,
                           hipStream_t s) {
 #define MFM_DT_ONLY(NF, DED)                                                    \
   hipLaunchKernelGGL((k_dt_only<Sys<NF, DED>>), dim3(L.nblk), dim3(kBlock), 0, \
-                     s, C, P, L.r, L.n, dx, dt_part, idx_part)
+                     s, CT, P, L.r, L.n, dt_part, idx_part)
   switch (C.nf * 2 + C.dedner) {
     case 2: MFM_DT_ONLY(1, 0); break;
     case 3: MFM_DT_ONLY(1, 1); break;
