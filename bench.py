@@ -3,7 +3,8 @@
 Workload (default): the 3-D multi-fluid heliospheric box, 256^3 cells on one
 patch (+4 ghost cells), plasma + 3 neutral fluids (23 components), solar wind
 from an inner sphere r = 0.1 into the LISM (26 km/s, 3 uG at 45 degrees),
-charge exchange on, Roe 8-wave fluxes, minmod, RK2, 8-wave sources.
+charge exchange on, Roe 8-wave fluxes, minmod, RK2, 8-wave sources, dt
+control of the reference deck (cfl 0.2, initial 0.1, growth 1.01).
 Inputs live in HBM before the timed region; each step is one full RK2 patch
 update (both stages, sources, floors and the CFL reduction) with the state
 advanced in place; dt follows the CFL condition from the previous step.
@@ -24,6 +25,10 @@ import numpy as np  # noqa: E402
 import mfmhd  # noqa: E402
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+
+# dt control of the reference's 3-D heliospheric deck (main/inputs/
+# helio03.inputs): cfl 0.2, initial_cfl 0.1, max_dt_growth 1.01
+CFL, CFL0, DT_GROWTH = 0.2, 0.1, 1.01
 
 # HBM bytes per k_update launch (average over the two RK2 stages) measured
 # with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this workload, scaled by
@@ -81,6 +86,65 @@ def cpu_baseline(n_sample, threads, budget_s=20.0):
                       "steps, OpenMP %d threads" % (n_sample, len(times), threads)}
 
 
+def serve_bench(prec, n_req=48, steps=20, rate_hz=400.0, seed=7):
+    """Ensemble serving: independent 3-D 4-fluid heliospheric solves with
+    grids n^3, n even in [16, 64], solar-wind speed ~ U[350, 800] km/s and LISM
+    field ~ U[2, 5] uG, arriving as a Poisson process, continuously batched
+    by mfm_server_* under an HBM budget.  Reports the p50/p99 per-request
+    step latency (one server iteration advances every active request one
+    RK2 step) beside the throughput."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(n_req):
+        n = 2 * int(rng.integers(8, 33))  # even: the sun sits on a node
+        sunV = float(rng.uniform(3.5e7, 8.0e7))
+        lismB = float(rng.uniform(2e-6, 5e-6))
+        dx = 1.0 / n
+        p = mfmhd.helio_params(lismN=0.06, lismV=2.6e6, lismT=6527.0, lismB=lismB,
+                               lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0),
+                               lismUdir=(-1.0, 0.0, 0.0), sunN=7.0, sunV=sunV,
+                               sunXYZ=(0.5, 0.5, 0.5), R0=0.15, initR=0.3,
+                               netN=0.1, dx=dx, num_cells=(n, n, n))
+        p.nfluids = 4
+        p.time_approx = mfmhd.TIME_RK2
+        p.flux = mfmhd.FLUX_ROE8
+        p.precision = prec
+        reqs.append((p, dx, n))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, size=n_req))
+    srv = mfmhd.Server(reqs[0][0], 64.0 * (1 << 30))
+    t0 = time.perf_counter()
+    nxt, ids, meta = 0, [], {}
+    while True:
+        now = time.perf_counter() - t0
+        while nxt < n_req and arrivals[nxt] <= now:
+            p, dx, n = reqs[nxt]
+            ids.append(srv.submit(p, dx, steps, CFL, CFL0, DT_GROWTH))
+            meta[ids[-1]] = (n, p.sun_v, p.lism_bx)
+            nxt += 1
+        na, npd, _ = srv.step()
+        for r in list(ids):
+            st = srv.info(r)["state"]
+            if st == mfmhd.REQ_FAILED:
+                raise RuntimeError("served request %s failed: %s" % (meta[r], srv.error(r)))
+            if st == mfmhd.REQ_DONE:
+                srv.release(r)
+                ids.remove(r)
+        if nxt == n_req and na == 0 and npd == 0 and not ids:
+            break
+        if na == 0 and npd == 0 and nxt < n_req:
+            time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return {"requests": n_req, "steps_per_request": steps,
+            "grid": "n^3, n even in [16, 64], 4 fluids (23 comps), RK2",
+            "arrival": "Poisson %.0f /s" % rate_hz,
+            "p50_step_ms": st["p50_ms"], "p99_step_ms": st["p99_ms"],
+            "iterations": st["iterations"],
+            "cell_updates_per_s": st["cell_updates"] / (st["busy_ms"] * 1e-3),
+            "wall_s": wall}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,9 +152,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--precision", choices=["fp32flux", "fp64"], default="fp32flux")
-    ap.add_argument("--cfl", type=float, default=0.8)
+    ap.add_argument("--cfl", type=float, default=CFL)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=40)
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the ensemble-serving measurement")
+    ap.add_argument("--serve-requests", type=int, default=48)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device once "
                          "(calibrates HBM byte counters under rocprofv3 --pmc)")
@@ -116,8 +183,14 @@ def main():
     valid = mfmhd.Box.of((0, 0, 0), (n - 1,) * 3)
     dtn, _ = ctx.compute_dt(Uf, valid, dx)
     t = 0.0
+    dt = None
+
+    def next_dt(dtn, dt):
+        # first step initial_cfl * dt_raw, then growth-limited cfl * dt_raw
+        return CFL0 * dtn if dt is None else min(args.cfl * dtn, DT_GROWTH * dt)
+
     for _ in range(args.warmup):
-        dt = args.cfl * dtn
+        dt = next_dt(dtn, dt)
         dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
         t += dt
     mfmhd.synchronize()
@@ -127,7 +200,7 @@ def main():
     e0.record()
     w0 = time.perf_counter()
     for _ in range(args.steps):
-        dt = args.cfl * dtn
+        dt = next_dt(dtn, dt)
         dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
         t += dt
     e1.record()
@@ -175,7 +248,8 @@ def main():
         "config": {"workload": "3-D multi-fluid heliospheric box %d^3, plasma + 3 "
                                "neutrals (23 comps), RK2, Roe8, minmod, 8-wave, "
                                "charge exchange, single patch" % n,
-                   "cells": int(cells), "components": nc, "precision": args.precision},
+                   "cells": int(cells), "components": nc, "precision": args.precision,
+                   "dt_control": "cfl %.2g, initial_cfl %.2g, max_dt_growth %.2f" % (args.cfl, CFL0, DT_GROWTH)},
         "roofline": {"bound": "hbm", "kernel": "k_update",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
@@ -197,6 +271,8 @@ def main():
         mfmhd.synchronize()
         a.free()
         b.free()
+    if not args.no_serve:
+        out["ensemble"] = serve_bench(prec, n_req=args.serve_requests)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cb = cpu_baseline(args.cpu_sample, threads)

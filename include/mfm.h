@@ -119,9 +119,10 @@ typedef struct mfm_params {
   double lism_bx, lism_by, lism_bz;
   double net_rho, net_p;
   double tmlim, reg_safe_z_top, reg_safe_z_bot;
-  double r0_dt_ignore;            /* reserved: the dt exclusion radius is   */
-                                  /* R0 - 2 dx of the calling level, as in  */
-                                  /* HeliosphericProblem::define (:490)     */
+  double r0_dt_ignore;            /* reserved (unused): the dt exclusion    */
+                                  /* radius is R0 - 2 dx of the dx passed   */
+                                  /* with each call, as m_R0dtIgnore in     */
+                                  /* HeliosphericProblem::define (:497)     */
   /* /chargeexcm/ (SETCHARGEEX_PARS, swlism/ChargeExchange2F.ChF:4-21)     */
   double ce_scale_vel;            /* lismV (cm/s)                         */
   double ce_sigma_lindsay;        /* 1e-16 * scaleLen * lismN             */
@@ -237,6 +238,48 @@ int mfm_init_problem_dev(mfm_ctx* ctx, mfm_fab U, mfm_box box, double dx,
 /* Periodic ghost fill of a single-box periodic level (the part of
  * LevelData::exchange that a one-box level needs).                         */
 int mfm_fill_periodic_dev(mfm_ctx* ctx, mfm_fab U, mfm_box valid, void* stream);
+
+/* ---- ensemble serving ----------------------------------------------------
+ * No counterpart in the reference, which runs one simulation per process.
+ * A server owns independent solves of one equation class (spacedim,
+ * nfluids, dedner, time scheme, precision of `cls`).  Each request brings
+ * its own parameters (problem constants and domain = its grid), dx, step
+ * count and CFL number.  mfm_server_step admits pending requests in arrival
+ * order while their device bytes (state + update scratch) fit the HBM
+ * budget and fewer than max_active run, generates their initial state on
+ * the device (mfm_init_problem_dev), then advances every active request by
+ * one step (its own dt, see mfm_server_submit) in the same batched launches
+ * (continuous batching) and retires those that reached their step count.
+ * A request's result is the same alone and in any mix.  A finished request
+ * keeps its state (and its share of the budget) until released.            */
+typedef struct mfm_server mfm_server;
+enum { MFM_REQ_PENDING = 0, MFM_REQ_ACTIVE = 1, MFM_REQ_DONE = 2, MFM_REQ_FAILED = 3 };
+int mfm_server_create(const mfm_params* cls, double hbm_budget_bytes,
+                      int max_active, mfm_server** out);
+int mfm_server_destroy(mfm_server* srv);
+const char* mfm_server_last_error(const mfm_server* srv);
+/* Step control as the AMR driver's (AMRLevelIdealMHD::computeNewDt,
+ * computeInitialDt): first dt = initial_cfl * dt_raw, then
+ * min(cfl * dt_raw, max_dt_growth * previous dt); dt_raw = dx / c_max. */
+int mfm_server_submit(mfm_server* srv, const mfm_params* p, double dx,
+                      int nsteps, double cfl, double initial_cfl,
+                      double max_dt_growth, long* id);
+/* One iteration; iter_ms = wall time of the batched step (0 if idle).      */
+int mfm_server_step(mfm_server* srv, int* n_active, int* n_pending,
+                    double* iter_ms);
+/* state: MFM_REQ_*; fab_box/ncomp describe the state FAB (valid + ghosts). */
+int mfm_server_request_info(mfm_server* srv, long id, int* state,
+                            int* steps_done, double* t, mfm_box* fab_box,
+                            int* ncomp);
+/* Why a request failed ("" otherwise; valid until the request is released). */
+const char* mfm_server_request_error(mfm_server* srv, long id);
+/* Copy the whole state FAB (component-major) of a request to host memory. */
+int mfm_server_fetch(mfm_server* srv, long id, double* host, size_t bytes);
+int mfm_server_release(mfm_server* srv, long id);
+/* Nearest-rank p50 / p99 of the iteration (= per-request step) latencies,
+ * iterations run, cell-updates done and the summed iteration time.         */
+int mfm_server_stats(mfm_server* srv, double* p50_ms, double* p99_ms,
+                     long* iterations, double* cell_updates, double* busy_ms);
 
 /* ---- device utilities for FFI callers without their own HIP runtime ----
  * (allocation, copies and stream-ordered timing on the library's runtime). */

@@ -26,6 +26,7 @@ UNITS = [
                               "-fno-hip-fp32-correctly-rounded-divide-sqrt"]),
     ("mfm_init.hip", ["-ffp-contract=off"]),
     ("mfm_capi.cpp", ["-ffp-contract=off"]),
+    ("mfm_serve.cpp", ["-ffp-contract=off"]),
 ]
 
 

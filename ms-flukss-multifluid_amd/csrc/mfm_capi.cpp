@@ -13,63 +13,17 @@
 
 #include "../../include/mfm.h"
 #include "mfm_device.hpp"
+#include "mfm_host.hpp"
 #include "mfm_kernels.hpp"
 
 using namespace mfm;
+using namespace mfm::host;
 
-namespace {
+namespace mfm {
+namespace host {
 
 constexpr int kBlk = 256;
 const double eos_mp = 1.67262171e-24, eos_k = 1.3806505e-16;
-
-struct HBox {
-  int lo[3], hi[3];
-  long n(int d) const { return (long)hi[d] - lo[d] + 1; }
-  long size() const {
-    for (int d = 0; d < 3; ++d)
-      if (hi[d] < lo[d]) return 0;
-    return n(0) * n(1) * n(2);
-  }
-};
-HBox hb(const mfm_box& b) {
-  HBox r;
-  for (int d = 0; d < 3; ++d) { r.lo[d] = b.lo[d]; r.hi[d] = b.hi[d]; }
-  return r;
-}
-HBox grow(HBox b, int g, int dims) {
-  for (int d = 0; d < dims; ++d) { b.lo[d] -= g; b.hi[d] += g; }
-  return b;
-}
-bool contains(const HBox& a, const HBox& b) {
-  for (int d = 0; d < 3; ++d)
-    if (b.lo[d] < a.lo[d] || b.hi[d] > a.hi[d]) return false;
-  return true;
-}
-
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-}  // namespace
-
-struct mfm_ctx {
-  mfm_params p;
-  Consts C;
-  int device = 0;
-  std::string err = "";
-  // device arena for scratch + tables
-  char* arena = nullptr;
-  size_t arena_bytes = 0;
-  // pinned host staging for tables and results
-  char* hbuf = nullptr;
-  size_t hbuf_bytes = 0;
-  size_t fsize() const { return p.precision == MFM_PREC_FP32_FLUX ? 4 : 8; }
-  // optional per-kernel timing, accumulated per stage and kernel kind
-  mfm::KernelTimer* timer = nullptr;
-  double kt_ms[2][mfm::KK_COUNT] = {};
-  long kt_n[2][mfm::KK_COUNT] = {};
-  double kt_items[2][mfm::KK_COUNT] = {};
-};
-
-namespace {
 
 int fail(mfm_ctx* c, int code, const std::string& m) {
   if (c) c->err = m;
@@ -226,15 +180,6 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
   C.sun_beta = p->problem == MFM_PROBLEM_HELIO ? p->omega * kAU / (p->sun_v * p->lism_v_dim) : 0.0;
   return MFM_OK;
 }
-
-struct PatchIn {
-  mfm_fab U;
-  mfm_box valid;
-  mfm_fab* F;
-  mfm_fab* divB;
-  int cidx;                // row of the constants table
-  double dt, dx, time;
-};
 
 // Geometry of one patch's scratch.
 struct PatchGeo {
@@ -549,7 +494,74 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
   return MFM_OK;
 }
 
-}  // namespace
+int run_dt(mfm_ctx* c, const Consts& C0, const mfm_fab& U, const mfm_box& box,
+           double dx, double* dt, int* min_cell, hipStream_t s) {
+  if (!c || !U.data || !dt) return MFM_ERR_ARG;
+  if (U.ncomp != C0.nc) return fail(c, MFM_ERR_SHAPE, "U.ncomp mismatch");
+  HBox ub = hb(U.box), b = hb(box);
+  if (!contains(ub, b)) return fail(c, MFM_ERR_SHAPE, "box outside U");
+  if (ub.size() >= (1L << 31)) return fail(c, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^31 cells");
+  if (!(dx > 0.0)) return fail(c, MFM_ERR_ARG, "dx must be positive");
+  RegionBuilder rb;
+  rb.add(0, b);
+  size_t off = 0;
+  size_t off_p = off; off += align256(sizeof(DevPatch));
+  size_t off_r = off; off += align256(sizeof(Region));
+  size_t off_c = off; off += align256(sizeof(Consts));
+  size_t off_dtp = off; off += align256(sizeof(double) * rb.nblk);
+  size_t off_idp = off; off += align256(sizeof(long) * rb.nblk);
+  size_t off_dto = off; off += align256(sizeof(double));
+  size_t off_ido = off; off += align256(sizeof(long));
+  int rc = ensure_arena(c, off);
+  if (!rc) rc = ensure_hbuf(c, 4096 + align256(sizeof(DevPatch)) + align256(sizeof(Region)) +
+                                 align256(sizeof(Consts)));
+  if (rc) return rc;
+  DevPatch* hp = reinterpret_cast<DevPatch*>(c->hbuf);
+  std::memset(hp, 0, sizeof(DevPatch));
+  hp->U = U.data;
+  for (int d = 0; d < 3; ++d) { hp->ulo[d] = ub.lo[d]; hp->un[d] = (int)ub.n(d); }
+  hp->ucs = ub.size();
+  hp->cidx = 0;
+  hp->dx = dx;
+  Region* hr = reinterpret_cast<Region*>(c->hbuf + off_r);
+  *hr = rb.v[0];
+  std::memcpy(c->hbuf + off_c, &C0, sizeof(Consts));
+  char* A = c->arena;
+  hipError_t e = hipMemcpyAsync(A + off_p, c->hbuf, off_c + sizeof(Consts), hipMemcpyHostToDevice, s);
+  RegionList L{reinterpret_cast<const Region*>(A + off_r), 1, rb.nblk, rb.items};
+  if (e == hipSuccess)
+    e = launch_dt_only(C0, reinterpret_cast<const Consts*>(A + off_c),
+                       reinterpret_cast<const DevPatch*>(A + off_p), L,
+                       reinterpret_cast<double*>(A + off_dtp), reinterpret_cast<long*>(A + off_idp), s);
+  if (e == hipSuccess)
+    e = launch_dt_final(L, reinterpret_cast<double*>(A + off_dtp), reinterpret_cast<long*>(A + off_idp),
+                        reinterpret_cast<double*>(A + off_dto), reinterpret_cast<long*>(A + off_ido), s);
+  double* hdt = reinterpret_cast<double*>(c->hbuf + off_c + align256(sizeof(Consts)));
+  long* hid = reinterpret_cast<long*>(c->hbuf + off_c + align256(sizeof(Consts)) + 256);
+  if (e == hipSuccess) e = hipMemcpyAsync(hdt, A + off_dto, sizeof(double), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(hid, A + off_ido, sizeof(long), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hipfail(c, e, "compute_dt");
+  double d = *hdt;
+  const Region& r = rb.v[0];
+  int cell[3] = {r.lo[0], r.lo[1], r.lo[2]};
+  if (!(d < std::numeric_limits<double>::infinity())) {
+    d = std::numeric_limits<double>::max();
+  } else {
+    long n01 = (long)r.n[0] * r.n[1];
+    long li = *hid;
+    cell[2] = r.lo[2] + (int)(li / n01);
+    cell[1] = r.lo[1] + (int)((li % n01) / r.n[0]);
+    cell[0] = r.lo[0] + (int)((li % n01) % r.n[0]);
+  }
+  *dt = d;
+  if (min_cell) { min_cell[0] = cell[0]; min_cell[1] = cell[1]; min_cell[2] = cell[2]; }
+  if (!(d > 0.0)) return fail(c, MFM_ERR_NUMERIC, "non-positive time step");
+  return MFM_OK;
+}
+
+}  // namespace host
+}  // namespace mfm
 
 extern "C" {
 
@@ -864,70 +876,8 @@ int mfm_patch_update_host(mfm_ctx* c, mfm_fab U, mfm_box valid, double dt,
 
 int mfm_patch_compute_dt_dev(mfm_ctx* c, mfm_fab U, mfm_box box, double dx,
                              double* dt, int min_cell[3], void* stream) {
-  if (!c || !U.data || !dt) return MFM_ERR_ARG;
-  const Consts& C0 = c->C;
-  if (U.ncomp != C0.nc) return fail(c, MFM_ERR_SHAPE, "U.ncomp mismatch");
-  HBox ub = hb(U.box), b = hb(box);
-  if (!contains(ub, b)) return fail(c, MFM_ERR_SHAPE, "box outside U");
-  if (ub.size() >= (1L << 31)) return fail(c, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^31 cells");
-  if (!(dx > 0.0)) return fail(c, MFM_ERR_ARG, "dx must be positive");
-  RegionBuilder rb;
-  rb.add(0, b);
-  size_t off = 0;
-  size_t off_p = off; off += align256(sizeof(DevPatch));
-  size_t off_r = off; off += align256(sizeof(Region));
-  size_t off_c = off; off += align256(sizeof(Consts));
-  size_t off_dtp = off; off += align256(sizeof(double) * rb.nblk);
-  size_t off_idp = off; off += align256(sizeof(long) * rb.nblk);
-  size_t off_dto = off; off += align256(sizeof(double));
-  size_t off_ido = off; off += align256(sizeof(long));
-  int rc = ensure_arena(c, off);
-  if (!rc) rc = ensure_hbuf(c, 4096 + align256(sizeof(DevPatch)) + align256(sizeof(Region)) +
-                                 align256(sizeof(Consts)));
-  if (rc) return rc;
-  hipStream_t s = (hipStream_t)stream;
-  DevPatch* hp = reinterpret_cast<DevPatch*>(c->hbuf);
-  std::memset(hp, 0, sizeof(DevPatch));
-  hp->U = U.data;
-  for (int d = 0; d < 3; ++d) { hp->ulo[d] = ub.lo[d]; hp->un[d] = (int)ub.n(d); }
-  hp->ucs = ub.size();
-  hp->cidx = 0;
-  hp->dx = dx;
-  Region* hr = reinterpret_cast<Region*>(c->hbuf + off_r);
-  *hr = rb.v[0];
-  std::memcpy(c->hbuf + off_c, &C0, sizeof(Consts));
-  char* A = c->arena;
-  hipError_t e = hipMemcpyAsync(A + off_p, c->hbuf, off_c + sizeof(Consts), hipMemcpyHostToDevice, s);
-  RegionList L{reinterpret_cast<const Region*>(A + off_r), 1, rb.nblk, rb.items};
-  if (e == hipSuccess)
-    e = launch_dt_only(C0, reinterpret_cast<const Consts*>(A + off_c),
-                       reinterpret_cast<const DevPatch*>(A + off_p), L,
-                       reinterpret_cast<double*>(A + off_dtp), reinterpret_cast<long*>(A + off_idp), s);
-  if (e == hipSuccess)
-    e = launch_dt_final(L, reinterpret_cast<double*>(A + off_dtp), reinterpret_cast<long*>(A + off_idp),
-                        reinterpret_cast<double*>(A + off_dto), reinterpret_cast<long*>(A + off_ido), s);
-  double* hdt = reinterpret_cast<double*>(c->hbuf + off_c + align256(sizeof(Consts)));
-  long* hid = reinterpret_cast<long*>(c->hbuf + off_c + align256(sizeof(Consts)) + 256);
-  if (e == hipSuccess) e = hipMemcpyAsync(hdt, A + off_dto, sizeof(double), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipMemcpyAsync(hid, A + off_ido, sizeof(long), hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) return hipfail(c, e, "compute_dt");
-  double d = *hdt;
-  const Region& r = rb.v[0];
-  int cell[3] = {r.lo[0], r.lo[1], r.lo[2]};
-  if (!(d < std::numeric_limits<double>::infinity())) {
-    d = std::numeric_limits<double>::max();
-  } else {
-    long n01 = (long)r.n[0] * r.n[1];
-    long li = *hid;
-    cell[2] = r.lo[2] + (int)(li / n01);
-    cell[1] = r.lo[1] + (int)((li % n01) / r.n[0]);
-    cell[0] = r.lo[0] + (int)((li % n01) % r.n[0]);
-  }
-  *dt = d;
-  if (min_cell) { min_cell[0] = cell[0]; min_cell[1] = cell[1]; min_cell[2] = cell[2]; }
-  if (!(d > 0.0)) return fail(c, MFM_ERR_NUMERIC, "non-positive time step");
-  return MFM_OK;
+  if (!c) return MFM_ERR_ARG;
+  return run_dt(c, c->C, U, box, dx, dt, min_cell, (hipStream_t)stream);
 }
 
 int mfm_init_problem_dev(mfm_ctx* c, mfm_fab U, mfm_box box, double dx,

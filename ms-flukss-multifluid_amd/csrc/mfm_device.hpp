@@ -229,6 +229,49 @@ __device__ __forceinline__ void cons_to_prim(const Consts& C, const double* u,
 template <typename T>
 struct MState { T p, rho, u, v, w, bx, by, bz; };
 
+// Differences (left - right) of the conserved variables turned into the
+// primitive-like jumps the left eigenvectors act on (src/Roe8WaveF.ChF,
+// the du block of RIEMANNF / RIEMANN7F).  With T = float this block runs in
+// FP64: in hypersonic flow the energy jump is a small difference of large
+// kinetic terms (|u| >> c) that FP32 does not resolve, and a wrong pressure
+// jump makes the whole characteristic decomposition wrong.
+template <typename T, bool seven>
+__device__ __forceinline__ void roe_jumps(const Consts& C, const MState<T>& L,
+                                          const MState<T>& R, T Rho_, T U_, T V_,
+                                          T W_, T BX_, T BY_, T BZ_, T kk_, T* out,
+                                          T* cons) {
+  using D = typename std::conditional<std::is_same<T, float>::value, double, T>::type;
+  const D half = D(0.5), c4 = D(k1_4PI), c8 = D(k1_8PI), hg = D(C.hgamma);
+  const D Lr = L.rho, Lu = L.u, Lv = L.v, Lw = L.w, Lp = L.p, Lbx = L.bx, Lby = L.by, Lbz = L.bz;
+  const D Rr = R.rho, Ru = R.u, Rv = R.v, Rw = R.w, Rp = R.p, Rbx = R.bx, Rby = R.by, Rbz = R.bz;
+  const D Rho = Rho_, U = U_, V = V_, W = W_, BX = BX_, BY = BY_, BZ = BZ_, kk = kk_;
+  const D RUL = Lr * Lu, RVL = Lr * Lv, RWL = Lr * Lw;
+  const D BxBL = c8 * (Lbx * Lbx + Lby * Lby + Lbz * Lbz);
+  const D RUxUL = half * (RUL * Lu + RVL * Lv + RWL * Lw);
+  const D EL = Lp / hg + RUxUL + BxBL;
+  const D RUR = Rr * Ru, RVR = Rr * Rv, RWR = Rr * Rw;
+  const D BxBR = c8 * (Rbx * Rbx + Rby * Rby + Rbz * Rbz);
+  const D RUxUR = half * (RUR * Ru + RVR * Rv + RWR * Rw);
+  const D ER = Rp / hg + RUxUR + BxBR;
+  D du1 = Lr - Rr, du2 = RUL - RUR, du3 = RVL - RVR, du4 = RWL - RWR;
+  D du5 = EL - ER;
+  const D du6 = Lbx - Rbx, du7 = Lby - Rby, du8 = Lbz - Rbz;
+  cons[0] = T(du1); cons[1] = T(du2); cons[2] = T(du3); cons[3] = T(du4);
+  cons[4] = T(du5); cons[5] = T(du6); cons[6] = T(du7); cons[7] = T(du8);
+  if constexpr (!seven) {
+    du5 = -hg * (-kk * du1 + U * du2 + V * du3 + W * du4 - du5 +
+                 (BX * du6 + BY * du7 + BZ * du8) * c4);
+  } else {
+    du5 = -hg * (-kk * du1 + U * du2 + V * du3 + W * du4 - du5 +
+                 (BY * du7 + BZ * du8) * c4);
+  }
+  du2 = (-U * du1 + du2) / Rho;
+  du3 = (-V * du1 + du3) / Rho;
+  du4 = (-W * du1 + du4) / Rho;
+  out[0] = T(du1); out[1] = T(du2); out[2] = T(du3); out[3] = T(du4);
+  out[4] = T(du5); out[5] = T(du6); out[6] = T(du7); out[7] = T(du8);
+}
+
 // 8-wave Roe flux (RIEMANNF, src/Roe8WaveF.ChF:5-601), left/right already
 // clamped to (smallp, smallr).  Only the non-zero eigenvector entries are
 // formed; the sums keep the reference's term order, so the FP64 build is
@@ -336,20 +379,10 @@ __device__ __forceinline__ void roe_flux(const Consts& C, const MState<T>& L,
   else { alf = tsqrt((cc - aas) / dfs); als = tsqrt((aaf - cc) / dfs); }
   const T sih = (BX >= zero) ? one : -one;
 
-  // differences (left - right) of conserved variables
-  T du1 = L.rho - R.rho, du2 = RUL - RUR, du3 = RVL - RVR, du4 = RWL - RWR;
-  T du5 = EL - ER;
-  const T du6 = L.bx - R.bx, du7 = L.by - R.by, du8 = L.bz - R.bz;
-  if constexpr (!seven) {
-    du5 = -hg * (-kk * du1 + U * du2 + V * du3 + W * du4 - du5 +
-                 (BX * du6 + BY * du7 + BZ * du8) * c4);
-  } else {
-    du5 = -hg * (-kk * du1 + U * du2 + V * du3 + W * du4 - du5 +
-                 (BY * du7 + BZ * du8) * c4);
-  }
-  du2 = (-U * du1 + du2) / Rho;
-  du3 = (-V * du1 + du3) / Rho;
-  du4 = (-W * du1 + du4) / Rho;
+  T du[8], dcons[8];
+  roe_jumps<T, seven>(C, L, R, Rho, U, V, W, BX, BY, BZ, kk, du, dcons);
+  const T du1 = du[0], du2 = du[1], du3 = du[2], du4 = du[3], du5 = du[4];
+  const T du6 = du[5], du7 = du[6], du8 = du[7];
   const T sqPiRho = tsqrt(T(kPI) * Rho), sq2PiRho = tsqrt(T(k2PI) * Rho);
 
   // non-zero entries of the left/right eigenvector matrices
@@ -370,19 +403,6 @@ __device__ __forceinline__ void roe_flux(const Consts& C, const MState<T>& L,
   const T so75 = half * als / (Rho * cc);
   const T so77 = -T(0.25) * alf * hyy / csp, so78 = -T(0.25) * alf * hzz / csp;
 
-  const T so15 = -ic;
-  const T lu1 = A1 * (du1 + so15 * du5);
-  const T lu3 = A3 * (-hzs * du3 + hys * du4 + q7 * du7 + (-q8) * du8);
-  const T lu4 = A4 * (-hzs * du3 + hys * du4 + (-q7) * du7 + q8 * du8);
-  const T lu5 = A5 * (so52 * du2 + so53 * du3 + so54 * du4 + so55 * du5 +
-                      so57 * du7 + so58 * du8);
-  const T lu6 = A6 * ((-so52) * du2 + (-so53) * du3 + (-so54) * du4 +
-                      so55 * du5 + so57 * du7 + so58 * du8);
-  const T lu7 = A7 * (so72 * du2 + so73 * du3 + so74 * du4 + so75 * du5 +
-                      so77 * du7 + so78 * du8);
-  const T lu8 = A8 * ((-so72) * du2 + (-so73) * du3 + (-so74) * du4 +
-                      so75 * du5 + so77 * du7 + so78 * du8);
-
   const T ra_f = Rho * alf, ra_s = Rho * als;
   const T sp35 = -ass * hyy * sih, sp37 = aff * hyy * sih;
   const T sp45 = -ass * hzz * sih, sp47 = aff * hzz * sih;
@@ -391,29 +411,99 @@ __device__ __forceinline__ void roe_flux(const Consts& C, const MState<T>& L,
   const T sp75 = two * als * sqPiRho * c * hyy, sp77 = -two * alf * sqPiRho * c * hyy;
   const T sp85 = two * als * sqPiRho * c * hzz, sp87 = -two * alf * sqPiRho * c * hzz;
 
-  T ra1 = lu1 + ra_f * lu5 + ra_f * lu6 + ra_s * lu7 + ra_s * lu8;
-  T ra2 = aff * lu5 + (-aff) * lu6 + ass * lu7 + (-ass) * lu8;
-  T ra3 = -hzs * lu3 + (-hzs) * lu4 + sp35 * lu5 + (-sp35) * lu6 + sp37 * lu7 +
-          (-sp37) * lu8;
-  T ra4 = hys * lu3 + hys * lu4 + sp45 * lu5 + (-sp45) * lu6 + sp47 * lu7 +
-          (-sp47) * lu8;
-  T ra5 = sp55 * lu5 + sp55 * lu6 + sp57 * lu7 + sp57 * lu8;
-  T ra7 = sp73 * lu3 + (-sp73) * lu4 + sp75 * lu5 + sp75 * lu6 + sp77 * lu7 +
-          sp77 * lu8;
-  T ra8 = sp83 * lu3 + (-sp83) * lu4 + sp85 * lu5 + sp85 * lu6 + sp87 * lu7 +
-          sp87 * lu8;
-  T ra6 = zero;
-  if constexpr (!seven) {
-    ra6 = A1 * du6;  // lu2 = |U| * du6, sp62 = 1
-    ra5 = ra1 * kk + Rho * (U * ra2 + V * ra3 + W * ra4) + ra5 / hg +
-          c4 * (BX * ra6 + BY * ra7 + BZ * ra8);
-  } else {
+  T ra1, ra2, ra3, ra4, ra5, ra6, ra7, ra8;
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32: |Lambda| = A1 I + D.  The A1 part is A1 times the conserved
+    // jumps (R L = I); only D = |Lambda| - A1 I goes through the
+    // eigenvectors.  With the reference's form, the entropy wave and the
+    // acoustic waves carry opposite dp/c^2 terms that cancel, which at
+    // large Mach number (c^2 << u^2, e.g. a pressure on its floor) FP32
+    // cannot resolve.  Pairs of waves are combined as half-sums and
+    // half-differences of D, each formed without cancellation.
+    const T del = T(C.del2) * (fabs(U) + fabs(V) + fabs(W) + af);
+    auto fix = [&](T e) -> T {
+      const T zz = fabs(e);
+      return zz >= del ? zz : half * (zz * zz + del * del) / del;
+    };
+    auto fixdiff = [&](T sh) -> T {  // fix(U + sh) - fix(U)
+      const T e1 = U + sh, z1 = fabs(e1), z0 = fabs(U);
+      if (z1 >= del && z0 >= del) {
+        if ((e1 >= zero) == (U >= zero)) return U >= zero ? sh : -sh;
+        return z1 - z0;
+      }
+      if (z1 < del && z0 < del) return half * sh * (U + U + sh) / del;
+      return fix(e1) - fix(U);
+    };
+    const T d3 = fixdiff(b), d4 = fixdiff(-b), d5 = fixdiff(af), d6 = fixdiff(-af);
+    const T d7 = fixdiff(as), d8 = fixdiff(-as);
+    const T Pa = -hzs * du3 + hys * du4, Qa = q7 * du7 + (-q8) * du8;
+    const T Xf = so52 * du2 + so53 * du3 + so54 * du4;
+    const T Yf = so55 * du5 + so57 * du7 + so58 * du8;
+    const T Xs = so72 * du2 + so73 * du3 + so74 * du4;
+    const T Ys = so75 * du5 + so77 * du7 + so78 * du8;
+    const T l3 = d3 * (Pa + Qa), l4 = d4 * (Pa - Qa);
+    const T s56 = (d5 - d6) * Xf + (d5 + d6) * Yf, m56 = (d5 + d6) * Xf + (d5 - d6) * Yf;
+    const T s78 = (d7 - d8) * Xs + (d7 + d8) * Ys, m78 = (d7 + d8) * Xs + (d7 - d8) * Ys;
+    ra1 = ra_f * s56 + ra_s * s78;
+    ra2 = aff * m56 + ass * m78;
+    ra3 = -hzs * (l3 + l4) + sp35 * m56 + sp37 * m78;
+    ra4 = hys * (l3 + l4) + sp45 * m56 + sp47 * m78;
+    ra5 = sp55 * s56 + sp57 * s78;
+    ra7 = sp73 * (l3 - l4) + sp75 * s56 + sp77 * s78;
+    ra8 = sp83 * (l3 - l4) + sp85 * s56 + sp87 * s78;
+    ra6 = zero;
     ra5 = ra1 * kk + Rho * (U * ra2 + V * ra3 + W * ra4) + ra5 / hg +
           c4 * (BY * ra7 + BZ * ra8);
+    ra2 = U * ra1 + Rho * ra2;
+    ra3 = V * ra1 + Rho * ra3;
+    ra4 = W * ra1 + Rho * ra4;
+    ra1 = ra1 + A1 * dcons[0];
+    ra2 = ra2 + A1 * dcons[1];
+    ra3 = ra3 + A1 * dcons[2];
+    ra4 = ra4 + A1 * dcons[3];
+    ra5 = ra5 + A1 * dcons[4];
+    if constexpr (!seven) ra6 = A1 * dcons[5];
+    ra7 = ra7 + A1 * dcons[6];
+    ra8 = ra8 + A1 * dcons[7];
+  } else {
+    const T so15 = -ic;
+    const T lu1 = A1 * (du1 + so15 * du5);
+    const T lu3 = A3 * (-hzs * du3 + hys * du4 + q7 * du7 + (-q8) * du8);
+    const T lu4 = A4 * (-hzs * du3 + hys * du4 + (-q7) * du7 + q8 * du8);
+    const T lu5 = A5 * (so52 * du2 + so53 * du3 + so54 * du4 + so55 * du5 +
+                        so57 * du7 + so58 * du8);
+    const T lu6 = A6 * ((-so52) * du2 + (-so53) * du3 + (-so54) * du4 +
+                        so55 * du5 + so57 * du7 + so58 * du8);
+    const T lu7 = A7 * (so72 * du2 + so73 * du3 + so74 * du4 + so75 * du5 +
+                        so77 * du7 + so78 * du8);
+    const T lu8 = A8 * ((-so72) * du2 + (-so73) * du3 + (-so74) * du4 +
+                        so75 * du5 + so77 * du7 + so78 * du8);
+
+    ra1 = lu1 + ra_f * lu5 + ra_f * lu6 + ra_s * lu7 + ra_s * lu8;
+    ra2 = aff * lu5 + (-aff) * lu6 + ass * lu7 + (-ass) * lu8;
+    ra3 = -hzs * lu3 + (-hzs) * lu4 + sp35 * lu5 + (-sp35) * lu6 + sp37 * lu7 +
+            (-sp37) * lu8;
+    ra4 = hys * lu3 + hys * lu4 + sp45 * lu5 + (-sp45) * lu6 + sp47 * lu7 +
+            (-sp47) * lu8;
+    ra5 = sp55 * lu5 + sp55 * lu6 + sp57 * lu7 + sp57 * lu8;
+    ra7 = sp73 * lu3 + (-sp73) * lu4 + sp75 * lu5 + sp75 * lu6 + sp77 * lu7 +
+            sp77 * lu8;
+    ra8 = sp83 * lu3 + (-sp83) * lu4 + sp85 * lu5 + sp85 * lu6 + sp87 * lu7 +
+            sp87 * lu8;
+    ra6 = zero;
+    if constexpr (!seven) {
+      ra6 = A1 * du6;  // lu2 = |U| * du6, sp62 = 1
+      ra5 = ra1 * kk + Rho * (U * ra2 + V * ra3 + W * ra4) + ra5 / hg +
+            c4 * (BX * ra6 + BY * ra7 + BZ * ra8);
+    } else {
+      ra5 = ra1 * kk + Rho * (U * ra2 + V * ra3 + W * ra4) + ra5 / hg +
+            c4 * (BY * ra7 + BZ * ra8);
+    }
+
+    ra2 = U * ra1 + Rho * ra2;
+    ra3 = V * ra1 + Rho * ra3;
+    ra4 = W * ra1 + Rho * ra4;
   }
-  ra2 = U * ra1 + Rho * ra2;
-  ra3 = V * ra1 + Rho * ra3;
-  ra4 = W * ra1 + Rho * ra4;
 
   const T fl1 = RUL, fr1 = RUR;
   const T fl2 = L.u * RUL - L.bx * L.bx * c4 + PL;

@@ -1,0 +1,82 @@
+// mfm_host.hpp -- host-side internals shared by the C ABI units: boxes,
+// the context, constants construction and the batched update driver.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/mfm.h"
+#include "mfm_device.hpp"
+#include "mfm_kernels.hpp"
+
+struct mfm_ctx {
+  mfm_params p;
+  mfm::Consts C;
+  int device = 0;
+  std::string err = "";
+  // device arena for scratch + tables
+  char* arena = nullptr;
+  size_t arena_bytes = 0;
+  // pinned host staging for tables and results
+  char* hbuf = nullptr;
+  size_t hbuf_bytes = 0;
+  size_t fsize() const { return p.precision == MFM_PREC_FP32_FLUX ? 4 : 8; }
+  // optional per-kernel timing, accumulated per stage and kernel kind
+  mfm::KernelTimer* timer = nullptr;
+  double kt_ms[2][mfm::KK_COUNT] = {};
+  long kt_n[2][mfm::KK_COUNT] = {};
+  double kt_items[2][mfm::KK_COUNT] = {};
+};
+
+namespace mfm {
+namespace host {
+
+// Inclusive cell box.
+struct HBox {
+  int lo[3], hi[3];
+  long n(int d) const { return (long)hi[d] - lo[d] + 1; }
+  long size() const {
+    for (int d = 0; d < 3; ++d)
+      if (hi[d] < lo[d]) return 0;
+    return n(0) * n(1) * n(2);
+  }
+};
+inline HBox hb(const mfm_box& b) {
+  HBox r;
+  for (int d = 0; d < 3; ++d) { r.lo[d] = b.lo[d]; r.hi[d] = b.hi[d]; }
+  return r;
+}
+inline HBox grow(HBox b, int g, int dims) {
+  for (int d = 0; d < dims; ++d) { b.lo[d] -= g; b.hi[d] += g; }
+  return b;
+}
+inline bool contains(const HBox& a, const HBox& b) {
+  for (int d = 0; d < 3; ++d)
+    if (b.lo[d] < a.lo[d] || b.hi[d] > a.hi[d]) return false;
+  return true;
+}
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int fail(mfm_ctx* c, int code, const std::string& m);
+int hipfail(mfm_ctx* c, hipError_t e, const char* what);
+int build_consts(const mfm_params* p, Consts& C, std::string& why);
+
+// One patch of a batched update.
+struct PatchIn {
+  mfm_fab U;
+  mfm_box valid;
+  mfm_fab* F;
+  mfm_fab* divB;
+  int cidx;                // row of the constants table
+  double dt, dx, time;
+};
+
+int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
+              double* new_dt, int* min_cell, double* level_dt, int* min_patch,
+              hipStream_t s);
+// computeDt of one box with the constants Cq.
+int run_dt(mfm_ctx* c, const Consts& Cq, const mfm_fab& U, const mfm_box& box,
+           double dx, double* dt, int* min_cell, hipStream_t s);
+
+}  // namespace host
+}  // namespace mfm

@@ -96,7 +96,13 @@ EXPORTS = [
     "mfm_dev_memset", "mfm_dev_synchronize", "mfm_event_create",
     "mfm_event_destroy", "mfm_event_record", "mfm_event_elapsed_ms",
     "mfm_ctx_set_timing", "mfm_ctx_get_timing",
+    "mfm_server_create", "mfm_server_destroy", "mfm_server_last_error",
+    "mfm_server_submit", "mfm_server_step", "mfm_server_request_info",
+    "mfm_server_fetch", "mfm_server_release", "mfm_server_stats",
+    "mfm_server_request_error",
 ]
+
+REQ_PENDING, REQ_ACTIVE, REQ_DONE, REQ_FAILED = range(4)
 
 # kernel kinds of mfm_ctx_get_timing
 KT_PRIM, KT_FLUX_PLASMA, KT_FLUX_NEUTRAL, KT_FLUX_BC, KT_UPDATE = range(5)
@@ -153,6 +159,21 @@ def load(path=LIB_PATH):
     lib.mfm_init_problem_dev.argtypes = [C.c_void_p, Fab, Box, C.c_double,
                                          C.c_void_p]
     lib.mfm_fill_periodic_dev.argtypes = [C.c_void_p, Fab, Box, C.c_void_p]
+    vp, dbl, lng, it = C.c_void_p, C.c_double, C.c_long, C.c_int
+    lib.mfm_server_create.argtypes = [P, dbl, it, C.POINTER(vp)]
+    lib.mfm_server_destroy.argtypes = [vp]
+    lib.mfm_server_last_error.argtypes = [vp]
+    lib.mfm_server_last_error.restype = C.c_char_p
+    lib.mfm_server_submit.argtypes = [vp, P, dbl, it, dbl, dbl, dbl, C.POINTER(lng)]
+    lib.mfm_server_step.argtypes = [vp, C.POINTER(it), C.POINTER(it), C.POINTER(dbl)]
+    lib.mfm_server_request_info.argtypes = [vp, lng, C.POINTER(it), C.POINTER(it),
+                                            C.POINTER(dbl), C.POINTER(Box), C.POINTER(it)]
+    lib.mfm_server_fetch.argtypes = [vp, lng, vp, C.c_size_t]
+    lib.mfm_server_release.argtypes = [vp, lng]
+    lib.mfm_server_request_error.argtypes = [vp, lng]
+    lib.mfm_server_request_error.restype = C.c_char_p
+    lib.mfm_server_stats.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(lng),
+                                     C.POINTER(dbl), C.POINTER(dbl)]
     lib.mfm_dev_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
     lib.mfm_dev_free.argtypes = [C.c_void_p]
     lib.mfm_dev_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
@@ -427,3 +448,83 @@ class Event:
             self.lib.mfm_event_destroy(self.h)
         except Exception:
             pass
+
+
+class Server:
+    """Ensemble serving over mfm_server_* (continuous batching of
+    independent solves of one equation class, admitted by an HBM budget)."""
+
+    def __init__(self, cls_params, hbm_budget_bytes, max_active=1 << 20):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.mfm_server_create(C.byref(cls_params), float(hbm_budget_bytes),
+                                        int(max_active), C.byref(h))
+        if rc != OK:
+            raise MfmError(rc, "mfm_server_create failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.mfm_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != OK:
+            msg = self.lib.mfm_server_last_error(self.h)
+            raise MfmError(rc, msg.decode() if msg else "")
+
+    def submit(self, params, dx, nsteps, cfl, initial_cfl=None, max_dt_growth=1.0e30):
+        """Queue a solve; dt control as the AMR driver's (first step
+        initial_cfl * dt_raw, then min(cfl * dt_raw, growth * last dt))."""
+        rid = C.c_long()
+        c0 = cfl if initial_cfl is None else initial_cfl
+        self._check(self.lib.mfm_server_submit(self.h, C.byref(params), float(dx),
+                                               int(nsteps), float(cfl), float(c0),
+                                               float(max_dt_growth), C.byref(rid)))
+        return rid.value
+
+    def step(self):
+        """One iteration: (n_active, n_pending, iteration ms)."""
+        na, npd, ms = C.c_int(), C.c_int(), C.c_double()
+        self._check(self.lib.mfm_server_step(self.h, C.byref(na), C.byref(npd), C.byref(ms)))
+        return na.value, npd.value, ms.value
+
+    def info(self, rid):
+        st, nd, t, nc = C.c_int(), C.c_int(), C.c_double(), C.c_int()
+        b = Box()
+        self._check(self.lib.mfm_server_request_info(self.h, rid, C.byref(st), C.byref(nd),
+                                                     C.byref(t), C.byref(b), C.byref(nc)))
+        lo, hi = tuple(b.lo), tuple(b.hi)
+        return {"state": st.value, "steps": nd.value, "t": t.value, "lo": lo, "hi": hi,
+                "ncomp": nc.value}
+
+    def fetch(self, rid):
+        """The request's state FAB as a numpy array (nc, nz, ny, nx)."""
+        import numpy as np
+        inf = self.info(rid)
+        lo, hi = inf["lo"], inf["hi"]
+        shape = (inf["ncomp"], hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+        a = np.empty(shape, dtype=np.float64)
+        self._check(self.lib.mfm_server_fetch(self.h, rid, C.c_void_p(a.ctypes.data), a.nbytes))
+        return a
+
+    def error(self, rid):
+        m = self.lib.mfm_server_request_error(self.h, rid)
+        return m.decode() if m else ""
+
+    def release(self, rid):
+        self._check(self.lib.mfm_server_release(self.h, rid))
+
+    def stats(self):
+        p50, p99, cu, busy = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        it = C.c_long()
+        self._check(self.lib.mfm_server_stats(self.h, C.byref(p50), C.byref(p99), C.byref(it),
+                                              C.byref(cu), C.byref(busy)))
+        return {"p50_ms": p50.value, "p99_ms": p99.value, "iterations": it.value,
+                "cell_updates": cu.value, "busy_ms": busy.value}

@@ -1802,11 +1802,13 @@ void euler_step(const Eq& E, Fab& U, Fab* Fout, Fab* divBout, double dt,
 }
 
 // PatchMHDMF::computeDt + PhysProblem/HeliosphericProblem::computeDt
+// (swlism/HeliosphericProblem.cpp:3809-3860); the exclusion radius is
+// m_R0dtIgnore = m_R0 - 2 dx of the level (:497).
 double compute_dt(const Eq& E, const Fab& U, const Box& box, double dx,
                   int cell[3]) {
   double dtMin = std::numeric_limits<double>::max();
   bool first = true;
-  double ign = E.P->problem == MFM_PROBLEM_HELIO ? E.P->r0_dt_ignore : -1.0;
+  double ign = E.P->problem == MFM_PROBLEM_HELIO ? E.P->sun_r0 - 2.0 * dx : -1.0;
   double ign2 = ign * ign;
   cell[0] = box.lo[0]; cell[1] = box.lo[1]; cell[2] = box.lo[2];
   FORBOX(box, i, j, k) {

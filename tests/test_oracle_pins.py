@@ -435,6 +435,44 @@ def test_compute_dt_brute_force(oracle_mod):
     assert dt == best and cell == bc
 
 
+def test_compute_dt_helio_excludes_inner_sphere(oracle_mod):
+    """HeliosphericProblem::computeDt skips cells within R0 - 2 dx of the
+    sun (swlism/HeliosphericProblem.cpp:3809-3860, m_R0dtIgnore at :497):
+    a fast spike at the sun does not set dt, one outside the radius does."""
+    from helpers import grid_shape
+    n = 16
+    dx = 1.0 / n
+    p = mfmhd.helio_params(sunXYZ=(0.5, 0.5, 0.5), R0=0.3, initR=0.4, netN=0.1,
+                           dx=dx, num_cells=(n, n, n))
+    p.nfluids = 2
+    vlo, vhi = (0, 0, 0), (n - 1,) * 3
+    lo, hi, shape = grid_shape(13, vlo, vhi, 4)
+    U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
+    r_ign = 0.3 - 2 * dx
+
+    def brute(V):
+        best, bc = None, None
+        for k in range(n):
+            for j in range(n):
+                for i in range(n):
+                    x, y, z = ((i + 0.5) * dx - 0.5, (j + 0.5) * dx - 0.5, (k + 0.5) * dx - 0.5)
+                    if x * x + y * y + z * z <= r_ign * r_ign:
+                        continue
+                    u = V[:, k - lo[2], j - lo[1], i - lo[0]]
+                    d = dx / oracle_mod.max_wave_speed(p, u)
+                    if best is None or d < best:
+                        best, bc = d, (i, j, k)
+        return best, bc
+
+    for (i, j, k) in ((8, 8, 8), (1, 2, 3)):
+        V = U.copy()
+        V[1, k - lo[2], j - lo[1], i - lo[0]] *= 1e4  # very fast cell
+        dt, cell = oracle_mod.compute_dt(p, V, lo, vlo, vhi, dx)
+        best, bc = brute(V)
+        assert dt == best and cell == bc
+        assert (cell == (i, j, k)) == ((i, j, k) == (1, 2, 3))
+
+
 def test_brio_wu_known_answer(oracle_mod):
     """Brio & Wu (1988) shock tube at t = 0.1 on 400 cells: undisturbed
     states outside the fans and the well-known plateaus (compound-wave
