@@ -229,18 +229,15 @@ __device__ __forceinline__ void cons_to_prim(const Consts& C, const double* u,
 template <typename T>
 struct MState { T p, rho, u, v, w, bx, by, bz; };
 
-// Differences (left - right) of the conserved variables turned into the
-// primitive-like jumps the left eigenvectors act on (src/Roe8WaveF.ChF,
-// the du block of RIEMANNF / RIEMANN7F).  With T = float this block runs in
-// FP64: in hypersonic flow the energy jump is a small difference of large
-// kinetic terms (|u| >> c) that FP32 does not resolve, and a wrong pressure
-// jump makes the whole characteristic decomposition wrong.
+// Differences (left - right) of the conserved variables (cons) and the
+// primitive-like jumps the left eigenvectors act on (out): the du block of
+// RIEMANNF / RIEMANN7F (src/Roe8WaveF.ChF).
 template <typename T, bool seven>
 __device__ __forceinline__ void roe_jumps(const Consts& C, const MState<T>& L,
                                           const MState<T>& R, T Rho_, T U_, T V_,
                                           T W_, T BX_, T BY_, T BZ_, T kk_, T* out,
                                           T* cons) {
-  using D = typename std::conditional<std::is_same<T, float>::value, double, T>::type;
+  using D = T;
   const D half = D(0.5), c4 = D(k1_4PI), c8 = D(k1_8PI), hg = D(C.hgamma);
   const D Lr = L.rho, Lu = L.u, Lv = L.v, Lw = L.w, Lp = L.p, Lbx = L.bx, Lby = L.by, Lbz = L.bz;
   const D Rr = R.rho, Ru = R.u, Rv = R.v, Rw = R.w, Rp = R.p, Rbx = R.bx, Rby = R.by, Rbz = R.bz;
