@@ -31,9 +31,10 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 CFL, CFL0, DT_GROWTH = 0.2, 0.1, 1.01
 
 # HBM bytes per k_update launch (average over the two RK2 stages) measured
-# with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on this workload, scaled by
-# the copy-kernel calibration; see profiles/k_update_traffic_256.txt.
-TRAFFIC = {}
+# with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the default workload
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 from a 1 GiB copy; summary in
+# profiles/k_update_helio256_traffic.md).  Valid for that workload only.
+TRAFFIC = {(256, "fp32flux"): 14052722890.0}
 
 
 def c3_params(n, prec):

@@ -2191,4 +2191,9 @@ double oracle_max_wave_speed(const mfm_params* p, const double* u) {
   return max_wave_speed_cell(E, u);
 }
 
+void oracle_floors(const mfm_params* p, double* u) {
+  Eq E(p, 1.0, 1.0);
+  floors_cell(E, u);
+}
+
 }  // extern "C"

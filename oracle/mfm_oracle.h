@@ -62,6 +62,8 @@ int oracle_region(const mfm_params* p, const double* w, int i, int j, int k,
                   double dx);
 /* MAXWAVESPEED + MAXWAVESPEED_E for one cell */
 double oracle_max_wave_speed(const mfm_params* p, const double* u);
+/* POSTPROCESSING / POSTPROCESSING_MF positivity floors on one cell (in place) */
+void oracle_floors(const mfm_params* p, double* u);
 
 #ifdef __cplusplus
 }

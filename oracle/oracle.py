@@ -61,6 +61,7 @@ def load():
     lib.oracle_region.restype = C.c_int
     lib.oracle_max_wave_speed.argtypes = [P, dp]
     lib.oracle_max_wave_speed.restype = C.c_double
+    lib.oracle_floors.argtypes = [P, dp]
     _lib = lib
     return lib
 
@@ -205,3 +206,9 @@ def region(p, w, i, j, k, dx):
 
 def max_wave_speed(p, u):
     return load().oracle_max_wave_speed(C.byref(p), _ptr(_vec(u)))
+
+
+def floors(p, u):
+    v = _vec(u)
+    load().oracle_floors(C.byref(p), _ptr(v))
+    return v[:len(u)].copy()

@@ -96,6 +96,104 @@ def test_cons_prim_floors(oracle_mod):
     assert w[8] == p.smallr
 
 
+@pytest.mark.parametrize("nf", [1, 4])
+def test_positivity_floors(oracle_mod, nf):
+    """POSTPROCESSING / POSTPROCESSING_MF (srcIdeal/PatchIdealMHDF.ChF:1019-1070,
+    srcIdeal/PatchMHDMFF.ChF:301-511): a fluid whose density or pressure is
+    below its floor gets the floor and a rebuilt energy (the multi-fluid
+    version keeps its velocity); the floors are smallr/smallp for the plasma
+    and neutral 1, 1e-6/1e-8 for neutral 2, 1e-7/1e-9 for neutral 3; a fluid
+    above its floors is left bit-for-bit unchanged."""
+    p = params(nf)
+    p.smallr, p.smallp = 1e-5, 1e-6
+    g = p.gamma
+    ok = np.array([1.0, 0.1, 0.2, 0.3, 0.5, 0.4, 0.3, 0.2])
+    fl = [ok]
+    for f in range(1, nf):
+        fl.append(np.array([0.7, 0.1, -0.2, 0.05, 0.3]))
+    u = prim_to_cons(p, np.concatenate(fl))
+    assert np.array_equal(oracle_mod.floors(p, u), u)
+    floors = {0: (p.smallr, p.smallp), 1: (p.smallr, p.smallp), 2: (1e-6, 1e-8),
+              3: (1e-7, 1e-9)}
+    for f in range(nf):
+        for which in ("rho", "p"):
+            v = u.copy()
+            b = 0 if f == 0 else 8 + 5 * (f - 1)
+            vel = v[b + 1:b + 4] / v[b]
+            if which == "rho":
+                v[b] = 0.1 * floors[f][0]
+                v[b + 1:b + 4] = v[b] * vel
+            else:
+                ke = 0.5 * v[b] * np.dot(vel, vel)
+                mag = np.dot(v[5:8], v[5:8]) / (8 * np.pi) if f == 0 else 0.0
+                v[b + 4] = ke + mag - 1.0  # negative pressure
+            out = oracle_mod.floors(p, v)
+            rho_f, p_f = floors[f]
+            assert out[b] == max(v[b], rho_f)
+            r = out[b]
+            pres = (g - 1) * (out[b + 4] - 0.5 * np.dot(out[b + 1:b + 4], out[b + 1:b + 4]) / r
+                              - (np.dot(out[5:8], out[5:8]) / (8 * np.pi) if f == 0 else 0.0))
+            assert pres == pytest.approx(max(p_f, pres), rel=1e-9, abs=1e-12)
+            if which == "p":
+                assert pres == pytest.approx(p_f, rel=1e-6)
+            if nf > 1 or f > 0:
+                assert np.allclose(out[b + 1:b + 4] / r, vel, rtol=1e-12)
+            # the other fluids are untouched
+            mask = np.ones(len(u), bool)
+            mask[b:b + 5] = False
+            if f == 0:
+                mask[:8] = False
+            assert np.array_equal(out[mask], v[mask])
+
+
+def test_helio_initial_state_invariants(oracle_mod):
+    """HELIOINIT (swlism/HeliosphericF.ChF:1150-1361): inside initR the
+    solar wind is radial with |v| = sunV, rho r^2 and p r^(2 gamma) constant;
+    outside, rho = 1 and velocity and field are the potential flow past a
+    sphere of radius initR: radial part (1 - (R/r)^3) (V.r), tangential part
+    (1 + (R/r)^3 / 2) V_t, for V the LISM vector.  Checked as properties of
+    the generated field."""
+    from helpers import grid_shape
+    n = 24
+    dx = 1.0 / n
+    p = mfmhd.helio_params(lismB=3e-6, lismBdir=(0.6, 0.8, 0.0), sunXYZ=(0.5, 0.5, 0.5),
+                           R0=0.1, initR=0.3, netN=0.1, sunV=4.5e7, dx=dx,
+                           num_cells=(n, n, n))
+    p.nfluids = 2
+    vlo, vhi = (0, 0, 0), (n - 1,) * 3
+    lo, hi, shape = grid_shape(13, vlo, vhi, 4)
+    U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
+    rho_r2, p_r2g, n_in, n_out = [], [], 0, 0
+    for k in range(n):
+        for j in range(n):
+            for i in range(n):
+                x, y, z = ((i + 0.5) * dx - 0.5, (j + 0.5) * dx - 0.5, (k + 0.5) * dx - 0.5)
+                r = np.sqrt(x * x + y * y + z * z)
+                u = U[:, k - lo[2], j - lo[1], i - lo[0]]
+                w = oracle_mod.cons_to_prim(p, u)
+                if 0.12 < r < 0.28:
+                    n_in += 1
+                    v = w[1:4]
+                    assert np.linalg.norm(v) == pytest.approx(p.sun_v, rel=1e-12)
+                    assert np.dot(v, (x, y, z)) / (np.linalg.norm(v) * r) == pytest.approx(1.0, abs=1e-12)
+                    rho_r2.append(w[0] * r * r)
+                    p_r2g.append(w[4] * r ** (2 * p.gamma))
+                elif r > 0.32:
+                    n_out += 1
+                    assert w[0] == pytest.approx(1.0, rel=1e-12)
+                    e = np.array((x, y, z)) / r
+                    q = (0.3 / r) ** 3
+                    for got, far in ((w[1:4], (p.lism_ux, p.lism_uy, p.lism_uz)),
+                                     (w[5:8], (p.lism_bx, p.lism_by, p.lism_bz))):
+                        far = np.array(far)
+                        assert np.dot(got, e) == pytest.approx((1 - q) * np.dot(far, e), abs=1e-12)
+                        tg, tf = got - np.dot(got, e) * e, far - np.dot(far, e) * e
+                        assert np.allclose(tg, (1 + 0.5 * q) * tf, rtol=1e-10, atol=1e-12)
+    assert n_in > 50 and n_out > 1000
+    assert np.ptp(rho_r2) <= 1e-10 * np.mean(rho_r2)
+    assert np.ptp(p_r2g) <= 1e-9 * np.mean(p_r2g)
+
+
 # ------------------------------------------------------------- MHD fluxes --
 @pytest.mark.parametrize("d", [0, 1, 2])
 @pytest.mark.parametrize("avg", [0, 1])
