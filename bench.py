@@ -146,6 +146,85 @@ def serve_bench(prec, n_req=48, steps=20, rate_hz=400.0, seed=7):
             "wall_s": wall}
 
 
+def amr_bench(prec, n_patches=3000, levels=3, steps=5, warmup=1, seed=11):
+    """Batched AMR patch tables: `levels` levels (ratio 2), about n_patches
+    patches in all with edges drawn from {8, 16, 32} (seeded), 4 ghost cells,
+    4 fluids, on the heliospheric state of each level's grid.  Each level is
+    one mfm_level_update_dev call (every kernel launched once for all its
+    patches); a step updates every level once.  Returns cell-updates/s over
+    all levels."""
+    rng = np.random.default_rng(seed)
+    per_level = n_patches // levels
+    ng = 4
+    nc = 23
+    lv = []
+    total_cells = 0
+    for L in range(levels):
+        # patches sit in 32^3 slots of a cube of side m slots around the sun
+        m = int(np.ceil(per_level ** (1.0 / 3.0)))
+        n_dom = 32 * m
+        dx = 1.0 / n_dom
+        p = mfmhd.helio_params(lismN=0.06, lismV=2.6e6, lismT=6527.0, lismB=3e-6,
+                               lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0),
+                               lismUdir=(-1.0, 0.0, 0.0), sunN=7.0, sunV=4.5e7,
+                               sunXYZ=(0.5, 0.5, 0.5), R0=0.1, initR=0.3,
+                               netN=0.1, dx=dx, num_cells=(n_dom,) * 3)
+        p.nfluids = 4
+        p.time_approx = mfmhd.TIME_RK2
+        p.flux = mfmhd.FLUX_ROE8
+        p.precision = prec
+        slots = rng.permutation(m ** 3)[:per_level]
+        edges = rng.choice([8, 16, 32], size=per_level)
+        boxes, offs, off = [], [], 0
+        for q in range(per_level):
+            sx, sy, sz = slots[q] % m, (slots[q] // m) % m, slots[q] // (m * m)
+            e = int(edges[q])
+            vlo = (32 * sx, 32 * sy, 32 * sz)
+            vhi = tuple(v + e - 1 for v in vlo)
+            boxes.append((vlo, vhi))
+            offs.append(off)
+            off += nc * (e + 2 * ng) ** 3
+            total_cells += e ** 3
+        buf = mfmhd.DeviceArray((off,))
+        ctx = mfmhd.Context(p)
+        table = []
+        for (vlo, vhi), o in zip(boxes, offs):
+            lo = tuple(v - ng for v in vlo)
+            hi = tuple(v + ng for v in vhi)
+            f = mfmhd.fab(buf.ptr + 8 * o, lo, hi, nc)
+            ctx.init_problem(f, mfmhd.Box.of(lo, hi), dx)
+            table.append((f, mfmhd.Box.of(vlo, vhi)))
+        lv.append({"ctx": ctx, "buf": buf, "table": table, "dx": dx, "dt": None,
+                   "dtn": None, "t": 0.0})
+    mfmhd.synchronize()
+
+    def step():
+        for L in lv:
+            if L["dtn"] is None:
+                L["dtn"] = min(L["ctx"].compute_dt(f, b, L["dx"])[0] for f, b in L["table"])
+            dt = CFL0 * L["dtn"] if L["dt"] is None else min(CFL * L["dtn"], DT_GROWTH * L["dt"])
+            ld, _, _, _ = L["ctx"].update_level(L["table"], dt, L["dx"], L["t"])
+            L["dt"], L["dtn"] = dt, ld
+            L["t"] += dt
+
+    for _ in range(warmup):
+        step()
+    mfmhd.synchronize()
+    e0, e1 = mfmhd.Event(), mfmhd.Event()
+    e0.record()
+    for _ in range(steps):
+        step()
+    e1.record()
+    mfmhd.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    for L in lv:
+        L["ctx"].close()
+        L["buf"].free()
+    return {"patches": per_level * levels, "levels": levels,
+            "edges": "8/16/32 (seeded)", "cells": int(total_cells),
+            "ms_per_step": ms, "cell_updates_per_s": total_cells / (ms * 1e-3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +238,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the ensemble-serving measurement")
     ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--no-amr", action="store_true",
+                    help="skip the batched AMR patch-table measurement")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device once "
                          "(calibrates HBM byte counters under rocprofv3 --pmc)")
@@ -274,6 +355,8 @@ def main():
         b.free()
     if not args.no_serve:
         out["ensemble"] = serve_bench(prec, n_req=args.serve_requests)
+    if not args.no_amr:
+        out["amr_batched"] = amr_bench(prec)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cb = cpu_baseline(args.cpu_sample, threads)
