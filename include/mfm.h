@@ -183,6 +183,12 @@ enum {
   MFM_KT_UPDATE = 4        /* flux divergence, sources, floors, CFL      */
 };
 int mfm_ctx_set_timing(mfm_ctx* ctx, int on);
+/* hipGraph replay (no counterpart in the reference): when on, the launch
+ * sequence of an update (every kernel of both stages and the dt reduction)
+ * is captured once and replayed while the patch table keeps its shape; the
+ * per-call values (dt, time, constants) travel in the tables the library
+ * copies before each replay.  Ignored while kernel timing is on.           */
+int mfm_ctx_set_graphs(mfm_ctx* ctx, int on);
 int mfm_ctx_get_timing(mfm_ctx* ctx, int stage, int kind, double* ms,
                        long* launches, double* items);
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -413,10 +414,6 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
   double* dtp = reinterpret_cast<double*>(A + off_dtp);
   long* idp = reinterpret_cast<long*>(A + off_idp);
   const Consts* ct = reinterpret_cast<const Consts*>(A + off_ct);
-  auto launch = [&](const StageLists& L, const StepArgs& S) -> hipError_t {
-    if (p.precision == MFM_PREC_FP32_FLUX) return launch_stage<float>(C0, ct, dp, L, S, dtp, idp, s, c->timer);
-    return launch_stage<double>(C0, ct, dp, L, S, dtp, idp, s, c->timer);
-  };
   auto stage = [&](int l0) {
     StageLists L;
     L.prim = rl(l0);
@@ -430,23 +427,70 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
   };
   if (c->timer) c->timer->n = 0;
   int n_stage1 = 0;
-  if (rk2) {
-    StepArgs S1{1.0, 0, 0, 1, 0, 0, 0, 1};
-    e = launch(stage(0), S1);
-    if (e != hipSuccess) return hipfail(c, e, "stage 1 launch");
-    if (c->timer) n_stage1 = c->timer->n;
-    StepArgs S2{0.5, 1, 1, 0, 1, 1, 1, 0};
-    e = launch(stage(8), S2);
-    if (e != hipSuccess) return hipfail(c, e, "stage 2 launch");
+  // the whole launch sequence of one update (both stages, dt reduction)
+  auto enqueue = [&](hipStream_t st) -> int {
+    auto launch = [&](const StageLists& L, const StepArgs& S) -> hipError_t {
+      if (p.precision == MFM_PREC_FP32_FLUX)
+        return launch_stage<float>(C0, ct, dp, L, S, dtp, idp, st, c->timer);
+      return launch_stage<double>(C0, ct, dp, L, S, dtp, idp, st, c->timer);
+    };
+    hipError_t e2;
+    if (rk2) {
+      StepArgs S1{1.0, 0, 0, 1, 0, 0, 0, 1};
+      e2 = launch(stage(0), S1);
+      if (e2 != hipSuccess) return hipfail(c, e2, "stage 1 launch");
+      if (c->timer) n_stage1 = c->timer->n;
+      StepArgs S2{0.5, 1, 1, 0, 1, 1, 1, 0};
+      e2 = launch(stage(8), S2);
+      if (e2 != hipSuccess) return hipfail(c, e2, "stage 2 launch");
+    } else {
+      StepArgs S{1.0, 0, 0, 0, 0, 2, 1, 0};
+      e2 = launch(stage(8), S);
+      if (e2 != hipSuccess) return hipfail(c, e2, "first-order launch");
+    }
+    RegionList U2 = rl(15);
+    e2 = launch_dt_final(U2, dtp, idp, reinterpret_cast<double*>(A + off_dto),
+                         reinterpret_cast<long*>(A + off_ido), st);
+    if (e2 != hipSuccess) return hipfail(c, e2, "dt reduction launch");
+    return MFM_OK;
+  };
+  if (c->graphs && !c->timer) {
+    // The kernels read every per-call value (dt, time, constants, tables)
+    // from the arena, refreshed by the copy above; the sequence itself only
+    // depends on the arena layout and the list sizes.
+    std::vector<long long> key = {(long long)(uintptr_t)A, (long long)off_patch,
+                                  (long long)off_ct, (long long)off_dtp, (long long)off_idp,
+                                  (long long)off_dto, (long long)off_ido, dims, rk2 ? 1 : 0,
+                                  p.precision, C0.nf, C0.dedner};
+    for (int l = 0; l < kLists; ++l) {
+      key.push_back((long long)off_reg[l]);
+      key.push_back((long long)lists[l]->v.size());
+      key.push_back((long long)lists[l]->nblk);
+    }
+    if (!c->gexec || key != c->gkey) {
+      if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+      if (!c->cap_stream) {
+        e = hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking);
+        if (e != hipSuccess) { c->cap_stream = nullptr; return hipfail(c, e, "hipStreamCreate"); }
+      }
+      e = hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeThreadLocal);
+      if (e != hipSuccess) return hipfail(c, e, "hipStreamBeginCapture");
+      int rc2 = enqueue(c->cap_stream);
+      hipGraph_t g = nullptr;
+      e = hipStreamEndCapture(c->cap_stream, &g);
+      if (rc2 != MFM_OK) { if (g) (void)hipGraphDestroy(g); return rc2; }
+      if (e != hipSuccess) return hipfail(c, e, "hipStreamEndCapture");
+      e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) { c->gexec = nullptr; return hipfail(c, e, "hipGraphInstantiate"); }
+      c->gkey = key;
+    }
+    e = hipGraphLaunch(c->gexec, s);
+    if (e != hipSuccess) return hipfail(c, e, "hipGraphLaunch");
   } else {
-    StepArgs S{1.0, 0, 0, 0, 0, 2, 1, 0};
-    e = launch(stage(8), S);
-    if (e != hipSuccess) return hipfail(c, e, "first-order launch");
+    int rc2 = enqueue(s);
+    if (rc2 != MFM_OK) return rc2;
   }
-  RegionList U2 = rl(15);
-  e = launch_dt_final(U2, dtp, idp, reinterpret_cast<double*>(A + off_dto),
-                      reinterpret_cast<long*>(A + off_ido), s);
-  if (e != hipSuccess) return hipfail(c, e, "dt reduction launch");
   double* hdt = reinterpret_cast<double*>(c->hbuf + table_bytes);
   long* hid = reinterpret_cast<long*>(c->hbuf + table_bytes + align256(sizeof(double) * n));
   e = hipMemcpyAsync(hdt, A + off_dto, sizeof(double) * n, hipMemcpyDeviceToHost, s);
@@ -760,6 +804,18 @@ int mfm_ctx_set_timing(mfm_ctx* c, int on) {
   return MFM_OK;
 }
 
+int mfm_ctx_set_graphs(mfm_ctx* c, int on) {
+  if (!c) return MFM_ERR_ARG;
+  c->graphs = on != 0;
+  if (!c->graphs && c->gexec) {
+    (void)hipDeviceSynchronize();
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->gkey.clear();
+  }
+  return MFM_OK;
+}
+
 int mfm_ctx_get_timing(mfm_ctx* c, int stage, int kind, double* ms, long* launches,
                        double* items) {
   if (!c || stage < 0 || stage > 1 || kind < 0 || kind >= mfm::KK_COUNT) return MFM_ERR_ARG;
@@ -773,6 +829,8 @@ int mfm_ctx_destroy(mfm_ctx* c) {
   if (!c) return MFM_OK;
   (void)hipDeviceSynchronize();
   if (c->timer) (void)mfm_ctx_set_timing(c, 0);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   if (c->arena) (void)hipFree(c->arena);
   if (c->hbuf) (void)hipHostFree(c->hbuf);
   delete c;

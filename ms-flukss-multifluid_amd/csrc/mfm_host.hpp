@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/mfm.h"
 #include "mfm_device.hpp"
@@ -26,6 +27,11 @@ struct mfm_ctx {
   double kt_ms[2][mfm::KK_COUNT] = {};
   long kt_n[2][mfm::KK_COUNT] = {};
   double kt_items[2][mfm::KK_COUNT] = {};
+  // optional hipGraph replay of the launch sequence (not with the timer)
+  bool graphs = false;
+  hipStream_t cap_stream = nullptr;   // private stream used for capture
+  hipGraphExec_t gexec = nullptr;
+  std::vector<long long> gkey;        // what the captured sequence depends on
 };
 
 namespace mfm {

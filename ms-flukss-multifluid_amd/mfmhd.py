@@ -95,7 +95,7 @@ EXPORTS = [
     "mfm_dev_count", "mfm_dev_alloc", "mfm_dev_free", "mfm_dev_memcpy",
     "mfm_dev_memset", "mfm_dev_synchronize", "mfm_event_create",
     "mfm_event_destroy", "mfm_event_record", "mfm_event_elapsed_ms",
-    "mfm_ctx_set_timing", "mfm_ctx_get_timing",
+    "mfm_ctx_set_timing", "mfm_ctx_get_timing", "mfm_ctx_set_graphs",
     "mfm_server_create", "mfm_server_destroy", "mfm_server_last_error",
     "mfm_server_submit", "mfm_server_step", "mfm_server_request_info",
     "mfm_server_fetch", "mfm_server_release", "mfm_server_stats",
@@ -140,6 +140,7 @@ def load(path=LIB_PATH):
     lib.mfm_ctx_set_dedner.argtypes = [C.c_void_p, C.c_double, C.c_double]
     lib.mfm_ctx_last_error.argtypes = [C.c_void_p]
     lib.mfm_ctx_set_timing.argtypes = [C.c_void_p, C.c_int]
+    lib.mfm_ctx_set_graphs.argtypes = [C.c_void_p, C.c_int]
     lib.mfm_ctx_get_timing.argtypes = [C.c_void_p, C.c_int, C.c_int,
                                        C.POINTER(C.c_double),
                                        C.POINTER(C.c_long),
@@ -291,6 +292,10 @@ class Context:
     def set_timing(self, on):
         """Bracket every kernel launch with HIP events (resets the sums)."""
         self._check(self.lib.mfm_ctx_set_timing(self.h, 1 if on else 0))
+
+    def set_graphs(self, on):
+        """Replay the captured launch sequence while the table keeps its shape."""
+        self._check(self.lib.mfm_ctx_set_graphs(self.h, 1 if on else 0))
 
     def timing(self, stage, kind):
         """(milliseconds, launches, work items) summed for one RK2 stage

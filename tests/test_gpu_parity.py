@@ -287,3 +287,58 @@ def test_shape_errors_leave_state_untouched():
     assert rc == mfmhd.ERR_SHAPE
     assert np.array_equal(g.numpy(), U)
     ctx.close()
+
+
+def test_graph_replay_matches_eager():
+    """hipGraph replay of the update (mfm_ctx_set_graphs) gives the eager
+    results bit for bit on every replay, with eager work and
+    synchronizations between replays, and recaptures when the table changes
+    shape (one patch, then a two-patch level, then one patch again)."""
+    n = 20
+    dx = 1.0 / n
+    p = mfmhd.helio_params(lismB=3e-6, lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0), lismV=2.6e6,
+                           sunN=7.0, sunV=4.5e7, sunXYZ=(0.5, 0.5, 0.5), R0=0.15,
+                           initR=0.3, netN=0.1, dx=dx, num_cells=(n, n, n))
+    p.nfluids = 4
+    p.time_approx = mfmhd.TIME_RK2
+    p.precision = mfmhd.PREC_FP32_FLUX
+    nc = mfmhd.num_components(p)
+    vlo, vhi = (0, 0, 0), (n - 1,) * 3
+    lo, hi, shape = grid_shape(nc, vlo, vhi, 4)
+    eager, graph = mfmhd.Context(p), mfmhd.Context(p)
+    graph.set_graphs(True)
+    Ue = mfmhd.DeviceArray(shape)
+    fe = mfmhd.fab(Ue.ptr, lo, hi, nc)
+    eager.init_problem(fe, mfmhd.Box.of(lo, hi), dx)
+    mfmhd.synchronize()
+    Ug = mfmhd.DeviceArray(shape, data=Ue.numpy())
+    fg = mfmhd.fab(Ug.ptr, lo, hi, nc)
+    # a second, smaller patch (sub-box of the domain) for the level table
+    vlo2, vhi2 = (2, 3, 4), (9, 10, 11)
+    lo2, hi2, shape2 = grid_shape(nc, vlo2, vhi2, 4)
+    U2e = mfmhd.DeviceArray(shape2)
+    f2e = mfmhd.fab(U2e.ptr, lo2, hi2, nc)
+    eager.init_problem(f2e, mfmhd.Box.of(lo2, hi2), dx)
+    mfmhd.synchronize()
+    U2g = mfmhd.DeviceArray(shape2, data=U2e.numpy())
+    f2g = mfmhd.fab(U2g.ptr, lo2, hi2, nc)
+    valid, valid2 = mfmhd.Box.of(vlo, vhi), mfmhd.Box.of(vlo2, vhi2)
+    dt0, _ = eager.compute_dt(fe, valid, dx)
+    dt, t = 0.2 * dt0, 0.0
+    for step in range(6):
+        if step in (2, 3):
+            le = eager.update_level([(fe, valid), (f2e, valid2)], dt, dx, t)
+            lg = graph.update_level([(fg, valid), (f2g, valid2)], dt, dx, t)
+            assert le == lg
+        else:
+            de = eager.update_state(fe, valid, dt, dx, t)
+            dg = graph.update_state(fg, valid, dt, dx, t)
+            assert de == dg
+        mfmhd.synchronize()
+        assert np.array_equal(Ue.numpy(), Ug.numpy()), step
+        assert np.array_equal(U2e.numpy(), U2g.numpy()), step
+        # eager work between replays
+        assert eager.compute_dt(fe, valid, dx) == graph.compute_dt(fg, valid, dx)
+        t += dt
+    eager.close()
+    graph.close()
