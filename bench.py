@@ -238,6 +238,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the ensemble-serving measurement")
     ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default work walk instead of autotuning it")
     ap.add_argument("--no-amr", action="store_true",
                     help="skip the batched AMR patch-table measurement")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -266,6 +268,8 @@ def main():
     dtn, _ = ctx.compute_dt(Uf, valid, dx)
     t = 0.0
     dt = None
+    # autotuner: work walk of the kernels, timed on a copy of the state
+    walk = ctx.autotune(Uf, valid, CFL0 * dtn, dx, reps=2) if not args.no_tune else ctx.walk()
 
     def next_dt(dtn, dt):
         # first step initial_cfl * dt_raw, then growth-limited cfl * dt_raw
@@ -331,7 +335,8 @@ def main():
                                "neutrals (23 comps), RK2, Roe8, minmod, 8-wave, "
                                "charge exchange, single patch" % n,
                    "cells": int(cells), "components": nc, "precision": args.precision,
-                   "dt_control": "cfl %.2g, initial_cfl %.2g, max_dt_growth %.2f" % (args.cfl, CFL0, DT_GROWTH)},
+                   "dt_control": "cfl %.2g, initial_cfl %.2g, max_dt_growth %.2f" % (args.cfl, CFL0, DT_GROWTH),
+                   "walk": {"row_tile": walk[0], "xcd_order": walk[1]}},
         "roofline": {"bound": "hbm", "kernel": "k_update",
                      "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,

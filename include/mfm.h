@@ -189,6 +189,18 @@ int mfm_ctx_set_timing(mfm_ctx* ctx, int on);
  * per-call values (dt, time, constants) travel in the tables the library
  * copies before each replay.  Ignored while kernel timing is on.           */
 int mfm_ctx_set_graphs(mfm_ctx* ctx, int on);
+/* Work walk of the kernels (no counterpart in the reference): rows of a
+ * region are visited in row_tile x row_tile tiles in (y, z), and with
+ * xcd_order = 1 every XCD receives a contiguous range of blocks.  Any value
+ * gives the same results; the walk only changes cache reuse.  Defaults 8, 1. */
+int mfm_ctx_set_walk(mfm_ctx* ctx, int row_tile, int xcd_order);
+int mfm_ctx_get_walk(const mfm_ctx* ctx, int* row_tile, int* xcd_order);
+/* Autotuner: times `reps` updates of a device copy of U (U itself is not
+ * modified) for row tiles {2, 4, 8, 16, 32} x xcd_order {0, 1}, keeps the
+ * fastest walk in the context and returns it with its ms per update.       */
+int mfm_ctx_autotune(mfm_ctx* ctx, mfm_fab U, mfm_box valid, double dt,
+                     double dx, double time, int reps, int* row_tile,
+                     int* xcd_order, double* best_ms);
 int mfm_ctx_get_timing(mfm_ctx* ctx, int stage, int kind, double* ms,
                        long* launches, double* items);
 

@@ -436,15 +436,15 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     };
     hipError_t e2;
     if (rk2) {
-      StepArgs S1{1.0, 0, 0, 1, 0, 0, 0, 1};
+      StepArgs S1{1.0, 0, 0, 1, 0, 0, 0, 1, c->walk_tile, c->walk_xcd};
       e2 = launch(stage(0), S1);
       if (e2 != hipSuccess) return hipfail(c, e2, "stage 1 launch");
       if (c->timer) n_stage1 = c->timer->n;
-      StepArgs S2{0.5, 1, 1, 0, 1, 1, 1, 0};
+      StepArgs S2{0.5, 1, 1, 0, 1, 1, 1, 0, c->walk_tile, c->walk_xcd};
       e2 = launch(stage(8), S2);
       if (e2 != hipSuccess) return hipfail(c, e2, "stage 2 launch");
     } else {
-      StepArgs S{1.0, 0, 0, 0, 0, 2, 1, 0};
+      StepArgs S{1.0, 0, 0, 0, 0, 2, 1, 0, c->walk_tile, c->walk_xcd};
       e2 = launch(stage(8), S);
       if (e2 != hipSuccess) return hipfail(c, e2, "first-order launch");
     }
@@ -461,7 +461,8 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     std::vector<long long> key = {(long long)(uintptr_t)A, (long long)off_patch,
                                   (long long)off_ct, (long long)off_dtp, (long long)off_idp,
                                   (long long)off_dto, (long long)off_ido, dims, rk2 ? 1 : 0,
-                                  p.precision, C0.nf, C0.dedner};
+                                  p.precision, C0.nf, C0.dedner, c->walk_tile,
+                                  c->walk_xcd};
     for (int l = 0; l < kLists; ++l) {
       key.push_back((long long)off_reg[l]);
       key.push_back((long long)lists[l]->v.size());
@@ -802,6 +803,71 @@ int mfm_ctx_set_timing(mfm_ctx* c, int on) {
       c->kt_items[st][k] = 0.0;
     }
   return MFM_OK;
+}
+
+int mfm_ctx_set_walk(mfm_ctx* c, int row_tile, int xcd_order) {
+  if (!c || row_tile < 1 || row_tile > 64) return MFM_ERR_ARG;
+  c->walk_tile = row_tile;
+  c->walk_xcd = xcd_order ? 1 : 0;
+  return MFM_OK;
+}
+
+int mfm_ctx_get_walk(const mfm_ctx* c, int* row_tile, int* xcd_order) {
+  if (!c) return MFM_ERR_ARG;
+  if (row_tile) *row_tile = c->walk_tile;
+  if (xcd_order) *xcd_order = c->walk_xcd;
+  return MFM_OK;
+}
+
+int mfm_ctx_autotune(mfm_ctx* c, mfm_fab U, mfm_box valid, double dt, double dx,
+                     double time, int reps, int* row_tile, int* xcd_order,
+                     double* best_ms) {
+  if (!c || !U.data || reps < 1) return MFM_ERR_ARG;
+  const HBox ub = hb(U.box);
+  const size_t bytes = (size_t)ub.size() * U.ncomp * sizeof(double);
+  double* tmp = nullptr;
+  hipError_t e = hipMalloc((void**)&tmp, bytes);
+  if (e != hipSuccess) return hipfail(c, e, "hipMalloc (autotune copy)");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = MFM_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = MFM_ERR_DEVICE;
+  mfm_fab T = U;
+  T.data = tmp;
+  const int tiles[] = {2, 4, 8, 16, 32};
+  int best_t = c->walk_tile, best_x = c->walk_xcd;
+  double best = std::numeric_limits<double>::infinity();
+  for (int x = 0; x < 2 && rc == MFM_OK; ++x)
+    for (int t : tiles) {
+      c->walk_tile = t;
+      c->walk_xcd = x;
+      // the same starting state for every candidate
+      e = hipMemcpy(tmp, U.data, bytes, hipMemcpyDeviceToDevice);
+      if (e != hipSuccess) { rc = hipfail(c, e, "autotune copy"); break; }
+      double nd;
+      int cell[3];
+      rc = mfm_patch_update_dev(c, T, valid, dt, dx, time, nullptr, nullptr, &nd, cell, nullptr);
+      if (rc != MFM_OK && rc != MFM_ERR_NUMERIC) break;
+      rc = MFM_OK;
+      (void)hipEventRecord(e0, nullptr);
+      for (int r = 0; r < reps && rc == MFM_OK; ++r) {
+        rc = mfm_patch_update_dev(c, T, valid, dt, dx, time, nullptr, nullptr, &nd, cell, nullptr);
+        if (rc == MFM_ERR_NUMERIC) rc = MFM_OK;
+      }
+      (void)hipEventRecord(e1, nullptr);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (rc == MFM_OK && ms / reps < best) { best = ms / reps; best_t = t; best_x = x; }
+    }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(tmp);
+  c->walk_tile = best_t;
+  c->walk_xcd = best_x;
+  if (row_tile) *row_tile = best_t;
+  if (xcd_order) *xcd_order = best_x;
+  if (best_ms) *best_ms = best;
+  return rc;
 }
 
 int mfm_ctx_set_graphs(mfm_ctx* c, int on) {

@@ -32,6 +32,9 @@ struct mfm_ctx {
   hipStream_t cap_stream = nullptr;   // private stream used for capture
   hipGraphExec_t gexec = nullptr;
   std::vector<long long> gkey;        // what the captured sequence depends on
+  // work walk of the region kernels (row tile in (y, z), XCD renumbering)
+  int walk_tile = 8;
+  int walk_xcd = 1;
 };
 
 namespace mfm {

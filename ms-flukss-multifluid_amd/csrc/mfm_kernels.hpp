@@ -61,6 +61,8 @@ struct StepArgs {
   int fo_mode;         // caller fluxes: 0 none, 1 average with stage 1, 2 copy
   int reduce_dt;       // compute the CFL reduction from the stage output
   int write_w;         // store W of the stage output (input of the next stage)
+  int row_tile;        // work walk: (y, z) tile of rows (see cell_of)
+  int xcd_order;       // work walk: 1 = renumber blocks per XCD
 };
 
 // Optional kernel timing: a pair of events brackets every launch on the

@@ -96,6 +96,7 @@ EXPORTS = [
     "mfm_dev_memset", "mfm_dev_synchronize", "mfm_event_create",
     "mfm_event_destroy", "mfm_event_record", "mfm_event_elapsed_ms",
     "mfm_ctx_set_timing", "mfm_ctx_get_timing", "mfm_ctx_set_graphs",
+    "mfm_ctx_set_walk", "mfm_ctx_get_walk", "mfm_ctx_autotune",
     "mfm_server_create", "mfm_server_destroy", "mfm_server_last_error",
     "mfm_server_submit", "mfm_server_step", "mfm_server_request_info",
     "mfm_server_fetch", "mfm_server_release", "mfm_server_stats",
@@ -141,6 +142,11 @@ def load(path=LIB_PATH):
     lib.mfm_ctx_last_error.argtypes = [C.c_void_p]
     lib.mfm_ctx_set_timing.argtypes = [C.c_void_p, C.c_int]
     lib.mfm_ctx_set_graphs.argtypes = [C.c_void_p, C.c_int]
+    lib.mfm_ctx_set_walk.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.mfm_ctx_get_walk.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    lib.mfm_ctx_autotune.argtypes = [C.c_void_p, Fab, Box, C.c_double, C.c_double,
+                                     C.c_double, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_double)]
     lib.mfm_ctx_get_timing.argtypes = [C.c_void_p, C.c_int, C.c_int,
                                        C.POINTER(C.c_double),
                                        C.POINTER(C.c_long),
@@ -296,6 +302,21 @@ class Context:
     def set_graphs(self, on):
         """Replay the captured launch sequence while the table keeps its shape."""
         self._check(self.lib.mfm_ctx_set_graphs(self.h, 1 if on else 0))
+
+    def set_walk(self, row_tile, xcd_order):
+        self._check(self.lib.mfm_ctx_set_walk(self.h, int(row_tile), int(xcd_order)))
+
+    def walk(self):
+        t, x = C.c_int(), C.c_int()
+        self._check(self.lib.mfm_ctx_get_walk(self.h, C.byref(t), C.byref(x)))
+        return t.value, x.value
+
+    def autotune(self, U, valid, dt, dx, time=0.0, reps=2):
+        """Pick the fastest work walk on a copy of U -> (row_tile, xcd, ms)."""
+        t, x, ms = C.c_int(), C.c_int(), C.c_double()
+        self._check(self.lib.mfm_ctx_autotune(self.h, U, valid, dt, dx, time, int(reps),
+                                              C.byref(t), C.byref(x), C.byref(ms)))
+        return t.value, x.value, ms.value
 
     def timing(self, stage, kind):
         """(milliseconds, launches, work items) summed for one RK2 stage

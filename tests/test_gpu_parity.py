@@ -342,3 +342,41 @@ def test_graph_replay_matches_eager():
         t += dt
     eager.close()
     graph.close()
+
+
+def test_work_walk_does_not_change_results():
+    """The work walk (row tile, XCD renumbering) only reorders blocks: every
+    walk gives the default result bit for bit, and the autotuner leaves U
+    untouched and keeps one of its candidates."""
+    n = 20
+    dx = 1.0 / n
+    p = mfmhd.helio_params(lismB=3e-6, lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0), lismV=2.6e6,
+                           sunN=7.0, sunV=4.5e7, sunXYZ=(0.5, 0.5, 0.5), R0=0.15,
+                           initR=0.3, netN=0.1, dx=dx, num_cells=(n, n, n))
+    p.nfluids = 4
+    p.time_approx = mfmhd.TIME_RK2
+    p.precision = mfmhd.PREC_FP32_FLUX
+    nc = mfmhd.num_components(p)
+    vlo, vhi = (0, 0, 0), (n - 1,) * 3
+    lo, hi, shape = grid_shape(nc, vlo, vhi, 4)
+    ctx = mfmhd.Context(p)
+    U = mfmhd.DeviceArray(shape)
+    f = mfmhd.fab(U.ptr, lo, hi, nc)
+    ctx.init_problem(f, mfmhd.Box.of(lo, hi), dx)
+    mfmhd.synchronize()
+    U0 = U.numpy()
+    valid = mfmhd.Box.of(vlo, vhi)
+    dt0, _ = ctx.compute_dt(f, valid, dx)
+    tile, xcd, ms = ctx.autotune(f, valid, 0.1 * dt0, dx, reps=1)
+    assert np.array_equal(U.numpy(), U0)
+    assert tile in (2, 4, 8, 16, 32) and xcd in (0, 1) and ms > 0
+    results = []
+    for walk in ((8, 1), (2, 0), (16, 1), (32, 0)):
+        ctx.set_walk(*walk)
+        V = mfmhd.DeviceArray(shape, data=U0)
+        fv = mfmhd.fab(V.ptr, lo, hi, nc)
+        results.append((ctx.update_state(fv, valid, 0.1 * dt0, dx), V.numpy()))
+    for r in results[1:]:
+        assert r[0] == results[0][0]
+        assert np.array_equal(r[1], results[0][1])
+    ctx.close()
