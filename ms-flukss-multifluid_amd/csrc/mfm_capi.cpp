@@ -143,6 +143,9 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
   C.divb_method = p->divb_method;
   C.problem = p->problem;
   for (int c = 0; c < kMaxComp; ++c) C.lim[c] = (int8_t)(c < 8 ? p->limiter[c] : MFM_LIM_MINMOD);
+  C.plasma_lim = C.lim[0];
+  for (int c = 1; c < 8 + C.dedner; ++c)
+    if (C.lim[c] != C.lim[0]) C.plasma_lim = -1;
   C.gamma = p->gamma;
   C.hgamma = p->gamma - 1.0;
   C.invhgamma = 1.0 / C.hgamma;

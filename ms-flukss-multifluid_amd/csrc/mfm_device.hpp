@@ -46,6 +46,7 @@ struct Consts {
   int iAveraging;
   int eight_wave, divb_method, problem;
   int8_t lim[kMaxComp];
+  int plasma_lim;     // limiter shared by every plasma variable (psi incl.), or -1
   double gamma, hgamma, invhgamma, smallr, smallp, del2, smallB;
   double ch, cp;
   int dom_lo[3], dom_hi[3], phys[3];  // phys[d]: non-periodic physical dir
