@@ -225,6 +225,121 @@ def amr_bench(prec, n_patches=3000, levels=3, steps=5, warmup=1, seed=11):
             "ms_per_step": ms, "cell_updates_per_s": total_cells / (ms * 1e-3)}
 
 
+def fill_bench(box=16, per_side=10, levels=3, reps=5):
+    """Ghost fill of a nested 3-level hierarchy (ratio 2): each level is a
+    cube of per_side^3 boxes of box^3 cells (1000 boxes of 16^3 by default,
+    3000 patches in all), level L+1 covering the central half of level L's
+    domain, 4 ghost cells, 4 fluids on the HELIOINIT state.  One fill is, per
+    level, fillInterp from the coarser level (time coefficient 0.5, old and
+    new coarse data) and the exchange -- the ghost fill that precedes the
+    updates of a step (srcIdeal/AMRLevelIdealMHD.cpp:701-772)."""
+    ng, nc, r = 4, 23, 2
+    side = box * per_side
+    lv = []
+    for L in range(levels):
+        n_dom = side * r ** L
+        off0 = 0 if L == 0 else (n_dom - side) // 2
+        dx = 1.0 / n_dom
+        p = mfmhd.helio_params(lismN=0.06, lismV=2.6e6, lismT=6527.0, lismB=3e-6,
+                               lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0),
+                               lismUdir=(-1.0, 0.0, 0.0), sunN=7.0, sunV=4.5e7,
+                               sunXYZ=(0.5, 0.5, 0.5), R0=0.1, initR=0.3,
+                               netN=0.1, dx=dx, num_cells=(n_dom,) * 3)
+        p.nfluids = 4
+        p.time_approx = mfmhd.TIME_RK2
+        boxes = []
+        for c in range(per_side):
+            for b in range(per_side):
+                for a in range(per_side):
+                    lo = (off0 + a * box, off0 + b * box, off0 + c * box)
+                    boxes.append((lo, tuple(v + box - 1 for v in lo)))
+        fcells = (box + 2 * ng) ** 3
+        buf = mfmhd.DeviceArray((len(boxes) * nc * fcells,))
+        old = mfmhd.DeviceArray((len(boxes) * nc * fcells,))
+        ctx = mfmhd.Context(p)
+        fabs, ofabs = [], []
+        for q, (lo, hi) in enumerate(boxes):
+            glo = tuple(v - ng for v in lo)
+            ghi = tuple(v + ng for v in hi)
+            f = mfmhd.fab(buf.ptr + 8 * q * nc * fcells, glo, ghi, nc)
+            ctx.init_problem(f, mfmhd.Box.of(glo, ghi), dx)
+            fabs.append(f)
+            ofabs.append(mfmhd.fab(old.ptr + 8 * q * nc * fcells, glo, ghi, nc))
+        old.copy_from(buf)
+        lv.append({"p": p, "ctx": ctx, "boxes": boxes, "fabs": fabs, "ofabs": ofabs,
+                   "buf": buf, "old": old, "dx": dx, "n_dom": n_dom})
+    mfmhd.synchronize()
+    t0 = time.perf_counter()
+    for L, d in enumerate(lv):
+        if L == 0:
+            d["plan"] = mfmhd.FillPlan(d["p"], d["boxes"], interp_radius=ng)
+        else:
+            c = lv[L - 1]
+            d["plan"] = mfmhd.FillPlan(d["p"], d["boxes"], c["boxes"], r, ng, c["dx"])
+    plan_s = (time.perf_counter() - t0) / levels
+
+    def fill(which):
+        for L, d in enumerate(lv):
+            if L > 0 and which in ("all", "interp"):
+                c = lv[L - 1]
+                d["ctx"].fill_interp(d["plan"], d["fabs"], c["ofabs"], c["fabs"], 0.5)
+            if which in ("all", "exchange"):
+                d["ctx"].exchange(d["plan"], d["fabs"])
+
+    res = {}
+    for which in ("exchange", "interp", "all"):
+        fill(which)
+        mfmhd.synchronize()
+        e0, e1 = mfmhd.Event(), mfmhd.Event()
+        e0.record()
+        for _ in range(reps):
+            fill(which)
+        e1.record()
+        mfmhd.synchronize()
+        res[which] = e0.elapsed_ms(e1) / reps
+    info = [d["plan"].info() for d in lv]
+    xc = sum(i["exchange_cells"] for i in info)
+    fcl = sum(i["interp_fine_cells"] for i in info)
+    ccl = sum(i["interp_coarse_cells"] for i in info)
+    # algorithmic bytes: exchange reads and writes each ghost cell once; the
+    # interpolation reads the coarse cells it gathers twice (old, new) and
+    # writes each fine cell once (the gathered ring is not counted)
+    xb = 2 * 8 * nc * xc
+    ib = 2 * 8 * nc * ccl + 8 * nc * fcl
+    for d in lv:
+        d["plan"].close()
+        d["ctx"].close()
+        d["buf"].free()
+        d["old"].free()
+    return {"levels": levels, "patches": per_side ** 3 * levels, "box": box,
+            "ghost": ng, "components": nc, "plan_build_s_per_level": plan_s,
+            "ms_per_fill": res["all"],
+            "exchange": {"ms": res["exchange"], "ghost_cells": xc,
+                         "GB_per_s": xb / (res["exchange"] * 1e-3) / 1e9},
+            "interp": {"ms": res["interp"], "fine_cells": fcl, "coarse_cells": ccl,
+                       "GB_per_s": ib / (res["interp"] * 1e-3) / 1e9}}
+
+
+def copy_ceiling(gib=4, reps=5):
+    """Device-to-device copy bandwidth (read + write bytes per second): the
+    HBM rate a streaming kernel can reach on this device."""
+    nb = gib << 30
+    a = mfmhd.DeviceArray((nb // 8,), zero=True)
+    b = mfmhd.DeviceArray((nb // 8,))
+    b.copy_from(a)
+    mfmhd.synchronize()
+    e0, e1 = mfmhd.Event(), mfmhd.Event()
+    e0.record()
+    for _ in range(reps):
+        b.copy_from(a)
+    e1.record()
+    mfmhd.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    a.free()
+    b.free()
+    return 2.0 * nb / (ms * 1e-3)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +357,8 @@ def main():
                     help="keep the default work walk instead of autotuning it")
     ap.add_argument("--no-amr", action="store_true",
                     help="skip the batched AMR patch-table measurement")
+    ap.add_argument("--no-fill", action="store_true",
+                    help="skip the level ghost-fill measurement")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB device-to-device once "
                          "(calibrates HBM byte counters under rocprofv3 --pmc)")
@@ -362,6 +479,9 @@ def main():
         out["ensemble"] = serve_bench(prec, n_req=args.serve_requests)
     if not args.no_amr:
         out["amr_batched"] = amr_bench(prec)
+    if not args.no_fill:
+        out["ghost_fill"] = fill_bench()
+    out["roofline"]["copy_ceiling_GBps"] = copy_ceiling() / 1e9
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cb = cpu_baseline(args.cpu_sample, threads)

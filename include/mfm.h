@@ -257,6 +257,63 @@ int mfm_init_problem_dev(mfm_ctx* ctx, mfm_fab U, mfm_box box, double dx,
  * LevelData::exchange that a one-box level needs).                         */
 int mfm_fill_periodic_dev(mfm_ctx* ctx, mfm_fab U, mfm_box valid, void* stream);
 
+/* ---- ghost fill of a level on the device --------------------------------
+ * The step before updateState in AMRLevelIdealMHD::step
+ * (srcIdeal/AMRLevelIdealMHD.cpp:701-772): coarse-fine interpolation of the
+ * ghost cells that no box of the level covers, then the same-level exchange.
+ *
+ * A plan is built once per layout, on the host, from the level's boxes (its
+ * problem domain and periodicity are those of `p`, the level's parameters)
+ * and, for a refined level, the boxes of the next coarser level:
+ *   - exchange: the Copier of LevelData<FArrayBox>::exchange for ghost
+ *     depth interp_radius (Chombo 3.2.4, not part of the reference tree):
+ *     every ghost cell of a box that lies in another box, or in a periodic
+ *     image of any box, receives that box's valid value;
+ *   - interpolation: PiecewiseLinearFillPatchMHDAM::define
+ *     (srcIdeal/PiecewiseLinearFillPatchMHDAM.cpp:118-378): the fine ghost
+ *     cells to interpolate, the coarse cells under them and, per direction,
+ *     whether their slope is centred or one-sided.
+ * Creating a plan touches no device.  It fails with MFM_ERR_UNSUPPORTED when
+ * the coarse boxes do not cover every coarse cell the interpolation reads
+ * (the layouts are not properly nested).
+ * coarse_boxes = NULL / ncoarse = 0 builds an exchange-only plan (level 0).
+ * ref_ratio >= 2; interp_radius = the level's ghost depth (m_numGhost).     */
+typedef struct mfm_fill_plan mfm_fill_plan;
+int mfm_fill_plan_create(const mfm_params* p, const mfm_box* boxes, int n,
+                         const mfm_box* coarse_boxes, int ncoarse,
+                         int ref_ratio, int interp_radius, double dx_coarse,
+                         mfm_fill_plan** out);
+int mfm_fill_plan_destroy(mfm_fill_plan* plan);
+const char* mfm_fill_plan_last_error(const mfm_fill_plan* plan);
+/* Sizes of the plan: ghost cells written by the exchange, fine cells and
+ * coarse cells of the interpolation, device bytes it holds once used.      */
+int mfm_fill_plan_info(const mfm_fill_plan* plan, long* exchange_cells,
+                       long* interp_fine_cells, long* interp_coarse_cells,
+                       size_t* device_bytes);
+/* LevelData::exchange (srcIdeal/AMRLevelIdealMHD.cpp:772).  U: n device
+ * FABs in plan order, each holding its box grown by interp_radius.         */
+int mfm_level_exchange_dev(mfm_ctx* ctx, mfm_fill_plan* plan, const mfm_fab* U,
+                           void* stream);
+/* PiecewiseLinearFillPatchMHDAM::fillInterp
+ * (srcIdeal/PiecewiseLinearFillPatchMHDAM.cpp:383-432): coarse data at the
+ * fine time, alpha*new + (1-alpha)*old (old only when alpha == 0 or new is
+ * NULL, new only when alpha == 1 or old is NULL), piecewise constant, then
+ * centred / one-sided slopes, the multidimensional limiter (interplimit_v,
+ * srcIdeal/FineInterpMHDAMF.ChF:235-385) and, with GLM cleaning, the div(B)
+ * slope correction (interp_divB_0, :449-555).  coarse_old / coarse_new:
+ * ncoarse device FABs in plan order (valid boxes at least).  Writes only
+ * the interpolated ghost cells of U.                                        */
+int mfm_level_fill_interp_dev(mfm_ctx* ctx, mfm_fill_plan* plan,
+                              const mfm_fab* U, const mfm_fab* coarse_old,
+                              const mfm_fab* coarse_new, double alpha,
+                              void* stream);
+/* The time coefficient of the fill (srcIdeal/AMRLevelIdealMHD.cpp:710-745):
+ * t_old = t_coarse_new - dt_coarse, alpha = (time - t_old) /
+ * (t_coarse_new - t_old), snapped to 0 or 1 within 0.04 * dt_fine /
+ * ref_ratio; MFM_ERR_ARG when it is outside [0, 1].                        */
+int mfm_fill_time_coef(double time, double t_coarse_new, double dt_coarse,
+                       double dt_fine, int ref_ratio, double* alpha);
+
 /* ---- ensemble serving ----------------------------------------------------
  * No counterpart in the reference, which runs one simulation per process.
  * A server owns independent solves of one equation class (spacedim,

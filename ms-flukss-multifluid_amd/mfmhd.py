@@ -100,7 +100,9 @@ EXPORTS = [
     "mfm_server_create", "mfm_server_destroy", "mfm_server_last_error",
     "mfm_server_submit", "mfm_server_step", "mfm_server_request_info",
     "mfm_server_fetch", "mfm_server_release", "mfm_server_stats",
-    "mfm_server_request_error",
+    "mfm_server_request_error", "mfm_fill_plan_create",
+    "mfm_fill_plan_destroy", "mfm_fill_plan_last_error", "mfm_fill_plan_info",
+    "mfm_level_exchange_dev", "mfm_level_fill_interp_dev", "mfm_fill_time_coef",
 ]
 
 REQ_PENDING, REQ_ACTIVE, REQ_DONE, REQ_FAILED = range(4)
@@ -181,6 +183,17 @@ def load(path=LIB_PATH):
     lib.mfm_server_request_error.restype = C.c_char_p
     lib.mfm_server_stats.argtypes = [vp, C.POINTER(dbl), C.POINTER(dbl), C.POINTER(lng),
                                      C.POINTER(dbl), C.POINTER(dbl)]
+    BP = C.POINTER(Box)
+    FP = C.POINTER(Fab)
+    lib.mfm_fill_plan_create.argtypes = [P, BP, it, BP, it, it, it, dbl, C.POINTER(vp)]
+    lib.mfm_fill_plan_destroy.argtypes = [vp]
+    lib.mfm_fill_plan_last_error.argtypes = [vp]
+    lib.mfm_fill_plan_last_error.restype = C.c_char_p
+    lib.mfm_fill_plan_info.argtypes = [vp, C.POINTER(lng), C.POINTER(lng), C.POINTER(lng),
+                                       C.POINTER(C.c_size_t)]
+    lib.mfm_level_exchange_dev.argtypes = [vp, vp, FP, vp]
+    lib.mfm_level_fill_interp_dev.argtypes = [vp, vp, FP, FP, FP, dbl, vp]
+    lib.mfm_fill_time_coef.argtypes = [dbl, dbl, dbl, dbl, it, C.POINTER(dbl)]
     lib.mfm_dev_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
     lib.mfm_dev_free.argtypes = [C.c_void_p]
     lib.mfm_dev_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
@@ -379,6 +392,73 @@ class Context:
     def fill_periodic(self, U, valid, stream=None):
         self._check(self.lib.mfm_fill_periodic_dev(
             self.h, U, valid, C.c_void_p(stream) if stream else None))
+
+    def exchange(self, plan, fabs, stream=None):
+        """LevelData::exchange of the level's FABs (in plan order)."""
+        arr = (Fab * len(fabs))(*fabs)
+        self._check(self.lib.mfm_level_exchange_dev(
+            self.h, plan.h, arr, C.c_void_p(stream) if stream else None))
+
+    def fill_interp(self, plan, fabs, coarse_old, coarse_new, alpha, stream=None):
+        """PiecewiseLinearFillPatchMHDAM::fillInterp of the level's ghost
+        cells from the coarser level (old and/or new data, either may be
+        None)."""
+        arr = (Fab * len(fabs))(*fabs)
+        co = (Fab * len(coarse_old))(*coarse_old) if coarse_old is not None else None
+        cn = (Fab * len(coarse_new))(*coarse_new) if coarse_new is not None else None
+        self._check(self.lib.mfm_level_fill_interp_dev(
+            self.h, plan.h, arr, co, cn, float(alpha),
+            C.c_void_p(stream) if stream else None))
+
+
+class FillPlan:
+    """Ghost-fill plan of a level layout (mfm_fill_plan_create): exchange
+    and, with coarse boxes, coarse-fine interpolation.  Host-only until first
+    used by Context.exchange / Context.fill_interp."""
+
+    def __init__(self, params, boxes, coarse_boxes=None, ref_ratio=2,
+                 interp_radius=4, dx_coarse=1.0):
+        self.lib = load()
+        fb = (Box * max(1, len(boxes)))(*[Box.of(lo, hi) for lo, hi in boxes])
+        cbl = coarse_boxes or []
+        cb = (Box * max(1, len(cbl)))(*[Box.of(lo, hi) for lo, hi in cbl])
+        h = C.c_void_p()
+        rc = self.lib.mfm_fill_plan_create(
+            C.byref(params), fb, len(boxes), cb if cbl else None, len(cbl),
+            int(ref_ratio), int(interp_radius), float(dx_coarse), C.byref(h))
+        self.h = h
+        if rc != OK:
+            msg = self.lib.mfm_fill_plan_last_error(h) if h else b""
+            self.close()
+            raise MfmError(rc, msg.decode() if msg else "")
+
+    def info(self):
+        xc, fc, cc, nb = C.c_long(), C.c_long(), C.c_long(), C.c_size_t()
+        self.lib.mfm_fill_plan_info(self.h, C.byref(xc), C.byref(fc), C.byref(cc),
+                                    C.byref(nb))
+        return {"exchange_cells": xc.value, "interp_fine_cells": fc.value,
+                "interp_coarse_cells": cc.value, "device_bytes": nb.value}
+
+    def close(self):
+        if self.h:
+            self.lib.mfm_fill_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fill_time_coef(time, t_coarse_new, dt_coarse, dt_fine, ref_ratio):
+    """alpha of AMRLevelIdealMHD::step (srcIdeal/AMRLevelIdealMHD.cpp:710-745)."""
+    a = C.c_double()
+    rc = load().mfm_fill_time_coef(float(time), float(t_coarse_new), float(dt_coarse),
+                                   float(dt_fine), int(ref_ratio), C.byref(a))
+    if rc != OK:
+        raise MfmError(rc, "time coefficient outside [0, 1]")
+    return a.value
 
 
 # ---- device memory and timing on the library's runtime ---------------------

@@ -35,6 +35,20 @@ int oracle_init_problem(const mfm_params* p, mfm_fab U, mfm_box box, double dx);
 /* Periodic ghost fill for a one-box periodic level. */
 int oracle_fill_periodic(const mfm_params* p, mfm_fab U, mfm_box valid);
 
+/* ---- level ghost fill (mfm_oracle_fill.cpp) ---------------------------- */
+/* LevelData::exchange of n boxes with ghost depth ng; U[i] holds box i grown
+ * by ng.  Domain and periodicity from p. */
+int oracle_level_exchange(const mfm_params* p, const mfm_box* boxes, int n,
+                          mfm_fab* U, int ng);
+/* PiecewiseLinearFillPatchMHDAM::define + fillInterp: fills the ghost cells
+ * of U[i] (boxes[i] grown by ng) from the coarse level (cboxes; cold / cnew
+ * either may be NULL) at time coefficient alpha.  p is the fine level. */
+int oracle_fill_interp(const mfm_params* p, const mfm_box* boxes, int n, mfm_fab* U,
+                       const mfm_box* cboxes, int ncb, const mfm_fab* cold,
+                       const mfm_fab* cnew, int r, int ng, double dxc, double alpha);
+/* interp_divB_0 on one slope triple (dBx/dx, dBy/dy, dBz/dz), in place */
+void oracle_interp_divb(double s[3], int dims);
+
 /* ---- pointwise kernels, for the pinning tests ------------------------- */
 /* Primitive -> face flux.  wl/wr: plasma primitive states in the face's
  * rotated order is NOT applied: wl/wr are (rho,u,v,w,p,bx,by,bz[,psi]) in

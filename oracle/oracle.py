@@ -62,6 +62,12 @@ def load():
     lib.oracle_max_wave_speed.argtypes = [P, dp]
     lib.oracle_max_wave_speed.restype = C.c_double
     lib.oracle_floors.argtypes = [P, dp]
+    BP = C.POINTER(mfmhd.Box)
+    F = C.POINTER(mfmhd.Fab)
+    lib.oracle_level_exchange.argtypes = [P, BP, C.c_int, F, C.c_int]
+    lib.oracle_fill_interp.argtypes = [P, BP, C.c_int, F, BP, C.c_int, F, F,
+                                       C.c_int, C.c_int, C.c_double, C.c_double]
+    lib.oracle_interp_divb.argtypes = [dp, C.c_int]
     _lib = lib
     return lib
 
@@ -212,3 +218,36 @@ def floors(p, u):
     v = _vec(u)
     load().oracle_floors(C.byref(p), _ptr(v))
     return v[:len(u)].copy()
+
+
+def _boxes(bl):
+    return (mfmhd.Box * max(1, len(bl)))(*[mfmhd.Box.of(lo, hi) for lo, hi in bl])
+
+
+def _fabs(arrs, los):
+    return (mfmhd.Fab * len(arrs))(*[fab_of(a, lo) for a, lo in zip(arrs, los)])
+
+
+def level_exchange(p, boxes, U, los, ng):
+    """LevelData::exchange on host FABs U[i] (numpy, (nc, nz, ny, nx)) whose
+    lower corners are los[i]; modifies U in place."""
+    rc = load().oracle_level_exchange(C.byref(p), _boxes(boxes), len(boxes),
+                                      _fabs(U, los), int(ng))
+    assert rc == 0, rc
+
+
+def fill_interp(p, boxes, U, los, cboxes, cold, cnew, clos, r, ng, dxc, alpha):
+    """PiecewiseLinearFillPatchMHDAM::fillInterp on host FABs (in place);
+    cold / cnew: lists of coarse FABs (or None)."""
+    co = _fabs(cold, clos) if cold is not None else None
+    cn = _fabs(cnew, clos) if cnew is not None else None
+    rc = load().oracle_fill_interp(C.byref(p), _boxes(boxes), len(boxes),
+                                   _fabs(U, los), _boxes(cboxes), len(cboxes), co, cn,
+                                   int(r), int(ng), float(dxc), float(alpha))
+    assert rc == 0, rc
+
+
+def interp_divb(s, dims=3):
+    v = np.array(s, dtype=np.float64)
+    load().oracle_interp_divb(_ptr(v), int(dims))
+    return v
