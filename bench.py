@@ -232,7 +232,9 @@ def fill_bench(box=16, per_side=10, levels=3, reps=5):
     domain, 4 ghost cells, 4 fluids on the HELIOINIT state.  One fill is, per
     level, fillInterp from the coarser level (time coefficient 0.5, old and
     new coarse data) and the exchange -- the ghost fill that precedes the
-    updates of a step (srcIdeal/AMRLevelIdealMHD.cpp:701-772)."""
+    updates of a step (srcIdeal/AMRLevelIdealMHD.cpp:701-772).  Also times
+    the coarse-fine synchronisation that follows them (flux register, reflux,
+    average down; :1282-1307, 1636-1647)."""
     ng, nc, r = 4, 23, 2
     side = box * per_side
     lv = []
@@ -266,6 +268,9 @@ def fill_bench(box=16, per_side=10, levels=3, reps=5):
             fabs.append(f)
             ofabs.append(mfmhd.fab(old.ptr + 8 * q * nc * fcells, glo, ghi, nc))
         old.copy_from(buf)
+        # the FAB tables of a level are built once, as a driver keeps its
+        # LevelData between steps
+        fabs, ofabs = mfmhd.fab_array(fabs), mfmhd.fab_array(ofabs)
         lv.append({"p": p, "ctx": ctx, "boxes": boxes, "fabs": fabs, "ofabs": ofabs,
                    "buf": buf, "old": old, "dx": dx, "n_dom": n_dom})
     mfmhd.synchronize()
@@ -297,6 +302,45 @@ def fill_bench(box=16, per_side=10, levels=3, reps=5):
         e1.record()
         mfmhd.synchronize()
         res[which] = e0.elapsed_ms(e1) / reps
+    # coarse-fine synchronisation of each level pair: register increments
+    # from the coarse fluxes and two fine steps, reflux, average down
+    fcells_face = (box + 1) * box * box
+    for d in lv:
+        d["F"] = mfmhd.DeviceArray((len(d["boxes"]) * 3 * nc * fcells_face,), zero=True)
+        d["Ff"] = []
+        for q, (lo, hi) in enumerate(d["boxes"]):
+            for dd in range(3):
+                fhi = list(hi)
+                fhi[dd] += 1
+                d["Ff"].append(mfmhd.fab(d["F"].ptr + 8 * (3 * q + dd) * nc * fcells_face,
+                                         lo, fhi, nc))
+    regs = [mfmhd.FluxRegister(lv[L]["p"], lv[L]["boxes"], lv[L - 1]["boxes"], r)
+            for L in range(1, levels)]
+    for d in lv:
+        d["Ff"] = mfmhd.fab_array(d["Ff"])
+
+    def sync():
+        for L in range(1, levels):
+            c, f, rg = lv[L - 1], lv[L], regs[L - 1]
+            rg.set_zero(c["ctx"])
+            rg.increment_coarse(c["ctx"], c["Ff"], 1e-3)
+            for _ in range(r):
+                rg.increment_fine(f["ctx"], f["Ff"], 5e-4)
+            rg.reflux(c["ctx"], c["fabs"], -1.0, c["dx"])
+            rg.average_down(c["ctx"], c["fabs"], f["fabs"], f["dx"])
+
+    sync()
+    mfmhd.synchronize()
+    e0, e1 = mfmhd.Event(), mfmhd.Event()
+    e0.record()
+    for _ in range(reps):
+        sync()
+    e1.record()
+    mfmhd.synchronize()
+    res["sync"] = e0.elapsed_ms(e1) / reps
+    rinfo = [rg.info() for rg in regs]
+    for rg in regs:
+        rg.close()
     info = [d["plan"].info() for d in lv]
     xc = sum(i["exchange_cells"] for i in info)
     fcl = sum(i["interp_fine_cells"] for i in info)
@@ -311,13 +355,19 @@ def fill_bench(box=16, per_side=10, levels=3, reps=5):
         d["ctx"].close()
         d["buf"].free()
         d["old"].free()
+        d["F"].free()
     return {"levels": levels, "patches": per_side ** 3 * levels, "box": box,
             "ghost": ng, "components": nc, "plan_build_s_per_level": plan_s,
             "ms_per_fill": res["all"],
             "exchange": {"ms": res["exchange"], "ghost_cells": xc,
                          "GB_per_s": xb / (res["exchange"] * 1e-3) / 1e9},
             "interp": {"ms": res["interp"], "fine_cells": fcl, "coarse_cells": ccl,
-                       "GB_per_s": ib / (res["interp"] * 1e-3) / 1e9}}
+                       "GB_per_s": ib / (res["interp"] * 1e-3) / 1e9},
+            "sync": {"ms": res["sync"],
+                     "register_faces": sum(i["faces"] for i in rinfo),
+                     "average_cells": sum(i["average_cells"] for i in rinfo),
+                     "what": "per level pair: register zero, coarse increment, 2 fine "
+                             "increments, reflux, average down"}}
 
 
 def copy_ceiling(gib=4, reps=5):

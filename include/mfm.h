@@ -277,7 +277,9 @@ int mfm_fill_periodic_dev(mfm_ctx* ctx, mfm_fab U, mfm_box valid, void* stream);
  * the coarse boxes do not cover every coarse cell the interpolation reads
  * (the layouts are not properly nested).
  * coarse_boxes = NULL / ncoarse = 0 builds an exchange-only plan (level 0).
- * ref_ratio >= 2; interp_radius = the level's ghost depth (m_numGhost).     */
+ * ref_ratio >= 2; interp_radius = the level's ghost depth (m_numGhost).
+ * The calls on one plan go to one stream (the plan keeps the FAB tables of
+ * its last calls on the device and reuses them while the FABs stay put).   */
 typedef struct mfm_fill_plan mfm_fill_plan;
 int mfm_fill_plan_create(const mfm_params* p, const mfm_box* boxes, int n,
                          const mfm_box* coarse_boxes, int ncoarse,
@@ -307,6 +309,56 @@ int mfm_level_fill_interp_dev(mfm_ctx* ctx, mfm_fill_plan* plan,
                               const mfm_fab* U, const mfm_fab* coarse_old,
                               const mfm_fab* coarse_new, double alpha,
                               void* stream);
+/* ---- coarse-fine synchronisation of a level pair ------------------------
+ * The step after the updates: the flux register of the coarse-fine
+ * interface and the average down (srcIdeal/AMRLevelIdealMHD.cpp:695,
+ * 1282-1307 and 1636-1647).  The register restates Chombo 3.2.4's
+ * LevelFluxRegister as MHDAM drives it (setToZero at the start of the coarse
+ * step, incrementCoarse(F, dt), incrementFine(F, dt) per fine step,
+ * reflux(U, -1, 1/dx)); the average is CoarseAverageExt::averageToCoarse
+ * (src/CoarseAverageExt.cpp:126-194, averagewithvolumes,
+ * src/CoarseAverageExtF.ChF:10-50).
+ *
+ * Built once per layout pair, on the host, from the fine level's boxes (its
+ * domain and periodicity from `p`, the fine level's parameters) and the
+ * coarse level's boxes.  Fine boxes must be coarsenable by ref_ratio.  A
+ * register face is a coarse face on the boundary of the coarsened fine
+ * region that is not on a non-periodic domain boundary; it belongs to the
+ * coarse cell outside the fine region.  After the increments a face holds
+ *   dF = sum over fine steps of dt_f <F_f> - dt_c F_c
+ * (<F_f> the mean of the r^(d-1) fine faces over it); reflux adds
+ *   scale * sigma * dF / dx_coarse
+ * to its cell, sigma = +1 when the face is the cell's high face, so that
+ * scale = -1 replaces the coarse flux through the interface by the fine
+ * fluxes (conservation across levels).  The calls on one register go to one
+ * stream.                                                                  */
+typedef struct mfm_flux_register mfm_flux_register;
+int mfm_flux_register_create(const mfm_params* p, const mfm_box* fine_boxes, int nfine,
+                             const mfm_box* coarse_boxes, int ncoarse, int ref_ratio,
+                             mfm_flux_register** out);
+int mfm_flux_register_destroy(mfm_flux_register* reg);
+const char* mfm_flux_register_last_error(const mfm_flux_register* reg);
+/* faces of the register, coarse cells reflux touches, coarse cells the
+ * average down writes */
+int mfm_flux_register_info(const mfm_flux_register* reg, long* faces, long* cells,
+                           long* average_cells);
+int mfm_flux_register_set_zero(mfm_ctx* ctx, mfm_flux_register* reg, void* stream);
+/* F: ncoarse * spacedim face FABs, F[box * spacedim + dir] = the fluxes of
+ * that coarse box in that direction (the F output of the update).          */
+int mfm_flux_register_increment_coarse(mfm_ctx* ctx, mfm_flux_register* reg,
+                                       const mfm_fab* F, double scale, void* stream);
+/* F: nfine * spacedim face FABs of the fine boxes, as above.               */
+int mfm_flux_register_increment_fine(mfm_ctx* ctx, mfm_flux_register* reg,
+                                     const mfm_fab* F, double scale, void* stream);
+/* U: ncoarse device FABs of the coarse level.                              */
+int mfm_flux_register_reflux(mfm_ctx* ctx, mfm_flux_register* reg, const mfm_fab* U,
+                             double scale, double dx_coarse, void* stream);
+/* Coarse cells under the fine boxes = volume-weighted mean of their fine
+ * children.  U_coarse: ncoarse FABs, U_fine: nfine FABs.                   */
+int mfm_level_average_down_dev(mfm_ctx* ctx, mfm_flux_register* reg,
+                               const mfm_fab* U_coarse, const mfm_fab* U_fine,
+                               double dx_fine, void* stream);
+
 /* The time coefficient of the fill (srcIdeal/AMRLevelIdealMHD.cpp:710-745):
  * t_old = t_coarse_new - dt_coarse, alpha = (time - t_old) /
  * (t_coarse_new - t_old), snapped to 0 or 1 within 0.04 * dt_fine /

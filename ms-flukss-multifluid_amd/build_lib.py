@@ -1,7 +1,8 @@
 """Build libmfmhd.so (HIP kernels for gfx950 + the C ABI) in-tree.
 
 Each translation unit is compiled with its own floating-point rules:
-  mfm_kernels_fp64.hip, mfm_init.hip, mfm_fill.hip : -ffp-contract=off
+  mfm_kernels_fp64.hip, mfm_init.hip, mfm_fill.hip, mfm_sync.hip :
+                                       -ffp-contract=off
                                        (reference rounding)
   mfm_kernels_fp32.hip               : FMA contraction, fast FP32 div/sqrt
 All HIP code is built with packed-FP32 instructions disabled.
@@ -29,6 +30,7 @@ UNITS = [
     ("mfm_capi.cpp", ["-ffp-contract=off"]),
     ("mfm_serve.cpp", ["-ffp-contract=off"]),
     ("mfm_fill.hip", ["-ffp-contract=off"]),
+    ("mfm_sync.hip", ["-ffp-contract=off"]),
     ("mfm_fill_plan.cpp", ["-ffp-contract=off"]),
 ]
 

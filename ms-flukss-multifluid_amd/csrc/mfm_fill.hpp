@@ -64,3 +64,43 @@ hipError_t launch_cell_copy(const GatherCell* G, long n, const FillFab* src_new,
 hipError_t launch_fill_interp(const InterpArgs& A, hipStream_t s);
 
 }  // namespace mfm
+
+namespace mfm {
+
+// One coarse face of a coarse-fine interface (flux register entry): the
+// coarse cell outside the fine region that owns it (box `cb`, cell c), the
+// direction, sigma = +1 when the face is the cell's high face, and the fine
+// box whose boundary faces cover it (first fine face f0; the others follow
+// along the transverse directions).
+struct RegFace {
+  int cb, d, sigma, fb;
+  int c[3];
+  int f0[3];
+};
+
+// The coarse cells of a reflux: entries [e0, e1) of the face list (grouped
+// by cell, in direction order, low face first).
+struct RegCell {
+  int e0, e1;
+};
+
+// A coarse cell of an average-down: fine box, coarse box, coarse cell.
+struct AvgRect {
+  int fb, cb;
+  int lo[3], n[3];
+  int blk0;
+};
+
+hipError_t launch_reg_zero(double* reg, long n, hipStream_t s);
+hipError_t launch_reg_coarse(const RegFace* E, long ne, const FillFab* F, int dims,
+                             double* reg, int nc, double scale, hipStream_t s);
+hipError_t launch_reg_fine(const RegFace* E, long ne, const FillFab* F, int dims, int r,
+                           double* reg, int nc, double scale, hipStream_t s);
+hipError_t launch_reflux(const RegCell* C, long ncell, const RegFace* E, long ne,
+                         const FillFab* U, const double* reg, int nc, double scale,
+                         double invvol, hipStream_t s);
+hipError_t launch_average_down(const AvgRect* R, int nr, int nblk, const FillFab* Uc,
+                               const FillFab* Uf, int nc, int dims, int r, double vol,
+                               hipStream_t s);
+
+}  // namespace mfm

@@ -124,6 +124,74 @@ struct BoxIndex {
   }
 };
 
+// Per-call FAB tables kept on the device, one slot per kind of call.  A
+// slot is rewritten only when its content changes (a level's FABs rarely
+// move), so repeated calls neither copy nor wait.  When it does change, the
+// host first waits for the slot's previous copy (its pinned buffer is
+// reused); the new copy is stream-ordered after the kernels that read the
+// slot before, which is why the calls on one plan use one stream.
+struct TableSlot {
+  FillFab* h = nullptr;
+  FillFab* d = nullptr;
+  size_t cap = 0;
+  std::vector<FillFab> last;
+  bool valid = false;
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+  void release() {
+    if (pending) (void)hipEventSynchronize(ev);
+    if (ev) (void)hipEventDestroy(ev);
+    if (h) (void)hipHostFree(h);
+    if (d) (void)hipFree(d);
+    h = nullptr; d = nullptr; ev = nullptr; cap = 0; pending = false; valid = false;
+  }
+};
+
+int slot_put(mfm_ctx* c, TableSlot& S, const std::vector<FillFab>& tab, hipStream_t st) {
+  if (S.valid && tab.size() == S.last.size() &&
+      (tab.empty() || std::memcmp(tab.data(), S.last.data(), tab.size() * sizeof(FillFab)) == 0))
+    return MFM_OK;
+  hipError_t e;
+  if (tab.size() > S.cap || !S.d) {
+    if (S.pending) (void)hipEventSynchronize(S.ev);
+    S.pending = false;
+    if (S.d) {
+      e = hipDeviceSynchronize();  // kernels may still read the old slot
+      if (e != hipSuccess) return mfm::host::hipfail(c, e, "hipDeviceSynchronize");
+      (void)hipFree(S.d);
+      S.d = nullptr;
+    }
+    if (S.h) { (void)hipHostFree(S.h); S.h = nullptr; }
+    const size_t cap = std::max<size_t>(tab.size(), 8);
+    e = hipMalloc((void**)&S.d, cap * sizeof(FillFab));
+    if (e != hipSuccess) { S.d = nullptr; return mfm::host::fail(c, MFM_ERR_NOMEM, "hipMalloc of a FAB table failed"); }
+    e = hipHostMalloc((void**)&S.h, cap * sizeof(FillFab), hipHostMallocDefault);
+    if (e != hipSuccess) { S.h = nullptr; return mfm::host::fail(c, MFM_ERR_NOMEM, "hipHostMalloc of a FAB table failed"); }
+    if (!S.ev) {
+      e = hipEventCreateWithFlags(&S.ev, hipEventDisableTiming);
+      if (e != hipSuccess) return mfm::host::hipfail(c, e, "hipEventCreate");
+    }
+    S.cap = cap;
+  }
+  if (S.pending) {
+    e = hipEventSynchronize(S.ev);
+    if (e != hipSuccess) return mfm::host::hipfail(c, e, "hipEventSynchronize");
+    S.pending = false;
+  }
+  S.valid = false;
+  if (!tab.empty()) {
+    std::memcpy(S.h, tab.data(), tab.size() * sizeof(FillFab));
+    e = hipMemcpyAsync(S.d, S.h, tab.size() * sizeof(FillFab), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return mfm::host::hipfail(c, e, "FAB table copy");
+    e = hipEventRecord(S.ev, st);
+    if (e != hipSuccess) return mfm::host::hipfail(c, e, "hipEventRecord");
+    S.pending = true;
+  }
+  S.last = tab;
+  S.valid = true;
+  return MFM_OK;
+}
+
 }  // namespace
 
 struct mfm_fill_plan {
@@ -152,10 +220,7 @@ struct mfm_fill_plan {
   FillPatch* d_fp = nullptr;
   double* d_work = nullptr;
   FillFab* d_wfab = nullptr;  // work FABs
-  FillFab* d_tab = nullptr;   // per call: fine, coarse new, coarse old
-  FillFab* h_tab = nullptr;   // pinned staging of d_tab
-  hipEvent_t tab_ev = nullptr;
-  bool tab_pending = false;
+  TableSlot tab[2];           // exchange: fine; interpolation: fine, coarse new, coarse old
 };
 
 namespace {
@@ -425,25 +490,16 @@ int upload(mfm_ctx* c, mfm_fill_plan* P) {
     return MFM_OK;
   }
   using mfm::host::align256;
-  const size_t ntab = P->fine.size() + 2 * P->crse.size();
   const size_t b_xr = align256(P->xr.size() * sizeof(CopyRect));
   const size_t b_gc = align256(P->gc.size() * sizeof(GatherCell));
   const size_t b_cells = align256(P->cells.size() * sizeof(int2));
   const size_t b_flags = align256(P->flags.size());
   const size_t b_fp = align256(P->fp.size() * sizeof(FillPatch));
   const size_t b_wfab = align256(P->fp.size() * sizeof(FillFab));
-  const size_t b_tab = align256(ntab * sizeof(FillFab));
   const size_t b_work = align256(P->work_doubles * sizeof(double));
-  const size_t total = b_xr + b_gc + b_cells + b_flags + b_fp + b_wfab + b_tab + b_work + 256;
+  const size_t total = b_xr + b_gc + b_cells + b_flags + b_fp + b_wfab + b_work + 256;
   e = hipMalloc((void**)&P->dmem, total);
   if (e != hipSuccess) { P->dmem = nullptr; return mfm::host::fail(c, MFM_ERR_NOMEM, "hipMalloc of the fill plan failed"); }
-  e = hipHostMalloc((void**)&P->h_tab, std::max<size_t>(b_tab, 256), hipHostMallocDefault);
-  if (e != hipSuccess) {
-    (void)hipFree(P->dmem); P->dmem = nullptr; P->h_tab = nullptr;
-    return mfm::host::fail(c, MFM_ERR_NOMEM, "hipHostMalloc of the fill plan failed");
-  }
-  e = hipEventCreateWithFlags(&P->tab_ev, hipEventDisableTiming);
-  if (e != hipSuccess) return mfm::host::hipfail(c, e, "hipEventCreate");
   P->device = dev;
   P->dbytes = total;
   char* q = P->dmem;
@@ -453,7 +509,6 @@ int upload(mfm_ctx* c, mfm_fill_plan* P) {
   P->d_flags = (uint8_t*)q; q += b_flags;
   P->d_fp = (FillPatch*)q; q += b_fp;
   P->d_wfab = (FillFab*)q; q += b_wfab;
-  P->d_tab = (FillFab*)q; q += b_tab;
   P->d_work = (double*)q;
   std::vector<FillFab> wf(P->fp.size());
   for (size_t i = 0; i < P->fp.size(); ++i) {
@@ -494,33 +549,23 @@ int to_fillfab(mfm_ctx* c, const mfm_fab& f, int nc, const HBox& need, int dims,
   return MFM_OK;
 }
 
-// Stage the per-call FAB tables: fine, then coarse new, then coarse old.
-int stage_tables(mfm_ctx* c, mfm_fill_plan* P, const mfm_fab* U,
+// Stage the per-call FAB tables into slot k: fine, then coarse new, then
+// coarse old (absent ones as empty entries).
+int stage_tables(mfm_ctx* c, mfm_fill_plan* P, int k, const mfm_fab* U,
                  const mfm_fab* cn, const mfm_fab* co, hipStream_t s) {
-  if (P->tab_pending) {
-    hipError_t e = hipEventSynchronize(P->tab_ev);
-    if (e != hipSuccess) return mfm::host::hipfail(c, e, "hipEventSynchronize");
-    P->tab_pending = false;
-  }
   const int n = (int)P->fine.size(), m = (int)P->crse.size();
+  std::vector<FillFab> t((size_t)n + (k == 1 ? 2 * (size_t)m : 0));
+  std::memset(t.data(), 0, t.size() * sizeof(FillFab));
   for (int i = 0; i < n; ++i) {
-    int rc = to_fillfab(c, U[i], P->nc, grow(P->fine[i], P->ng, P->dims), P->dims, P->h_tab[i], "U");
+    int rc = to_fillfab(c, U[i], P->nc, grow(P->fine[i], P->ng, P->dims), P->dims, t[i], "U");
     if (rc) return rc;
   }
-  size_t cnt = n;
-  if (cn || co) {
+  if (k == 1)
     for (int j = 0; j < m; ++j) {
-      if (cn) { int rc = to_fillfab(c, cn[j], P->nc, P->crse[j], P->dims, P->h_tab[n + j], "coarse_new"); if (rc) return rc; }
-      if (co) { int rc = to_fillfab(c, co[j], P->nc, P->crse[j], P->dims, P->h_tab[n + m + j], "coarse_old"); if (rc) return rc; }
+      if (cn) { int rc = to_fillfab(c, cn[j], P->nc, P->crse[j], P->dims, t[n + j], "coarse_new"); if (rc) return rc; }
+      if (co) { int rc = to_fillfab(c, co[j], P->nc, P->crse[j], P->dims, t[n + m + j], "coarse_old"); if (rc) return rc; }
     }
-    cnt = n + 2 * (size_t)m;
-  }
-  hipError_t e = hipMemcpyAsync(P->d_tab, P->h_tab, cnt * sizeof(FillFab), hipMemcpyHostToDevice, s);
-  if (e != hipSuccess) return mfm::host::hipfail(c, e, "fill table copy");
-  e = hipEventRecord(P->tab_ev, s);
-  if (e != hipSuccess) return mfm::host::hipfail(c, e, "hipEventRecord");
-  P->tab_pending = true;
-  return MFM_OK;
+  return slot_put(c, P->tab[k], t, s);
 }
 
 }  // namespace
@@ -572,13 +617,11 @@ int mfm_fill_plan_create(const mfm_params* p, const mfm_box* boxes, int n,
 
 int mfm_fill_plan_destroy(mfm_fill_plan* P) {
   if (!P) return MFM_OK;
-  if (P->tab_pending) (void)hipEventSynchronize(P->tab_ev);
-  if (P->tab_ev) (void)hipEventDestroy(P->tab_ev);
   if (P->dmem) {
     (void)hipDeviceSynchronize();
     (void)hipFree(P->dmem);
   }
-  if (P->h_tab) (void)hipHostFree(P->h_tab);
+  for (TableSlot& t : P->tab) t.release();
   delete P;
   return MFM_OK;
 }
@@ -598,7 +641,6 @@ int mfm_fill_plan_info(const mfm_fill_plan* P, long* xc, long* fc, long* cc,
     *bytes = align256(P->xr.size() * sizeof(CopyRect)) + align256(P->gc.size() * sizeof(GatherCell)) +
              align256(P->cells.size() * sizeof(int2)) + align256(P->flags.size()) +
              align256(P->fp.size() * sizeof(FillPatch)) + align256(P->fp.size() * sizeof(FillFab)) +
-             align256((P->fine.size() + 2 * P->crse.size()) * sizeof(FillFab)) +
              align256(P->work_doubles * sizeof(double)) + 256;
   }
   return MFM_OK;
@@ -613,10 +655,11 @@ int mfm_level_exchange_dev(mfm_ctx* c, mfm_fill_plan* P, const mfm_fab* U,
   hipStream_t s = (hipStream_t)stream;
   int rc = upload(c, P);
   if (rc) return rc;
-  rc = stage_tables(c, P, U, nullptr, nullptr, s);
+  rc = stage_tables(c, P, 0, U, nullptr, nullptr, s);
   if (rc) return rc;
-  hipError_t e = mfm::launch_rect_copy(P->d_xr, (int)P->xr.size(), P->xr_blk, P->d_tab,
-                                       P->d_tab, P->d_tab, P->nc, 1.0, 0.0, 0, s);
+  const FillFab* T = P->tab[0].d;
+  hipError_t e = mfm::launch_rect_copy(P->d_xr, (int)P->xr.size(), P->xr_blk, T, T, T, P->nc,
+                                       1.0, 0.0, 0, s);
   if (e != hipSuccess) return mfm::host::hipfail(c, e, "exchange launch");
   return MFM_OK;
 }
@@ -640,11 +683,12 @@ int mfm_level_fill_interp_dev(mfm_ctx* c, mfm_fill_plan* P, const mfm_fab* U,
   hipStream_t s = (hipStream_t)stream;
   int rc = upload(c, P);
   if (rc) return rc;
-  rc = stage_tables(c, P, U, mode == 1 ? nullptr : cnew, mode == 0 ? nullptr : cold, s);
+  rc = stage_tables(c, P, 1, U, mode == 1 ? nullptr : cnew, mode == 0 ? nullptr : cold, s);
   if (rc) return rc;
   const int n = (int)P->fine.size(), m = (int)P->crse.size();
-  hipError_t e = mfm::launch_cell_copy(P->d_gc, (long)P->gc.size(), P->d_tab + n,
-                                       P->d_tab + n + m, P->d_wfab, P->nc, alpha,
+  const FillFab* T = P->tab[1].d;
+  hipError_t e = mfm::launch_cell_copy(P->d_gc, (long)P->gc.size(), T + n, T + n + m,
+                                       P->d_wfab, P->nc, alpha,
                                        1.0 - alpha, mode, s);
   if (e != hipSuccess) return mfm::host::hipfail(c, e, "coarse gather launch");
   mfm::InterpArgs A;
@@ -653,7 +697,7 @@ int mfm_level_fill_interp_dev(mfm_ctx* c, mfm_fill_plan* P, const mfm_fab* U,
   A.ncell = (long)P->cells.size();
   A.P = P->d_fp;
   A.work = P->d_work;
-  A.U = P->d_tab;
+  A.U = T;
   A.nc = P->nc;
   A.dims = P->dims;
   A.r = P->r;
@@ -679,6 +723,373 @@ int mfm_fill_time_coef(double time, double tnew, double dtc, double dtf,
   if (std::fabs(1.0 - a) < eps) a = 1.0;
   if (!(a >= 0.0) || a > 1.0) return MFM_ERR_ARG;
   *alpha = a;
+  return MFM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Coarse-fine synchronisation of a level pair: the flux register of the
+// interface (LevelFluxRegister of Chombo 3.2.4 as MHDAM drives it:
+// setToZero at the coarse step, incrementCoarse / incrementFine with dt,
+// reflux with scale -1 and 1/dx, srcIdeal/AMRLevelIdealMHD.cpp:695,
+// 1282-1307, 1641) and the average down (CoarseAverageExt,
+// src/CoarseAverageExt.cpp:126-194).
+//
+// Register faces are the coarse faces on the boundary of the coarsened fine
+// region that do not lie on a non-periodic domain boundary; each is kept by
+// the coarse cell outside the fine region (after periodic wrapping), so
+// reflux touches only cells of the coarse level.  After the increments a
+// face holds dF = sum over fine steps of dt_f <F_f> - dt_c F_c, <F_f> the
+// mean of the r^(d-1) fine faces covering it; reflux with scale -1 adds
+// -sigma dF / dx to its cell (sigma = +1 on the cell's high face), which
+// replaces the coarse flux through the interface by the fine fluxes.
+
+using mfm::AvgRect;
+using mfm::RegCell;
+using mfm::RegFace;
+
+struct mfm_flux_register {
+  int dims = 3, nc = 0, r = 2;
+  bool ok = false;
+  std::string err;
+  std::vector<HBox> fine, crse;
+  std::vector<RegFace> faces;
+  std::vector<RegCell> cells;
+  std::vector<AvgRect> avg;
+  int avg_blk = 0;
+  long avg_cells = 0;
+  int device = -1;
+  char* dmem = nullptr;
+  RegFace* d_faces = nullptr;
+  RegCell* d_cells = nullptr;
+  AvgRect* d_avg = nullptr;
+  double* d_reg = nullptr;
+  TableSlot tab[4];  // coarse fluxes, fine fluxes, coarse U, coarse + fine U
+};
+
+namespace {
+
+int rfail(mfm_flux_register* R, int code, const std::string& m) {
+  R->err = m;
+  return code;
+}
+
+int build_register(mfm_flux_register* R, const mfm_params* p, const mfm_box* fb, int nf,
+                   const mfm_box* cbx, int ncb) {
+  using mfm::host::hb;
+  const int dims = R->dims, r = R->r;
+  HBox D;
+  int per[3] = {0, 0, 0};
+  for (int d = 0; d < 3; ++d) {
+    D.lo[d] = p->domain_lo[d];
+    D.hi[d] = p->domain_hi[d];
+    if (d < dims) per[d] = p->periodic[d] != 0;
+  }
+  if (empty(D)) return rfail(R, MFM_ERR_ARG, "empty problem domain");
+  for (int d = 0; d < dims; ++d)
+    if (fdiv(D.lo[d], r) * r != D.lo[d] || (D.hi[d] + 1) % r != 0)
+      return rfail(R, MFM_ERR_SHAPE, "problem domain is not a refinement of a coarse domain");
+  const HBox Dc = coarsen(D, r, dims);
+  int Lc[3] = {1, 1, 1};
+  for (int d = 0; d < dims; ++d) Lc[d] = (int)Dc.n(d);
+  std::vector<HBox> cf;
+  for (int i = 0; i < nf; ++i) {
+    const HBox b = hb(fb[i]);
+    if (empty(b)) return rfail(R, MFM_ERR_SHAPE, "empty fine box");
+    for (int d = 0; d < dims; ++d) {
+      if (b.lo[d] < D.lo[d] || b.hi[d] > D.hi[d])
+        return rfail(R, MFM_ERR_SHAPE, "fine box outside the problem domain");
+      if (fdiv(b.lo[d], r) * r != b.lo[d] || (b.hi[d] + 1) % r != 0)
+        return rfail(R, MFM_ERR_SHAPE, "fine box is not coarsenable by the refinement ratio");
+    }
+    R->fine.push_back(b);
+    cf.push_back(coarsen(b, r, dims));
+  }
+  for (int j = 0; j < ncb; ++j) {
+    const HBox b = hb(cbx[j]);
+    if (empty(b)) return rfail(R, MFM_ERR_SHAPE, "empty coarse box");
+    for (int d = 0; d < dims; ++d)
+      if (b.lo[d] < Dc.lo[d] || b.hi[d] > Dc.hi[d])
+        return rfail(R, MFM_ERR_SHAPE, "coarse box outside the coarse domain");
+    R->crse.push_back(b);
+  }
+  BoxIndex FI, CI;
+  FI.build(cf, dims);
+  CI.build(R->crse, dims);
+  std::vector<int> hits;
+  for (int i = 0; i < nf; ++i) {
+    FI.query(cf[i], hits);
+    if (hits.size() != 1) return rfail(R, MFM_ERR_SHAPE, "fine boxes overlap");
+  }
+  for (int j = 0; j < ncb; ++j) {
+    CI.query(R->crse[j], hits);
+    if (hits.size() != 1) return rfail(R, MFM_ERR_SHAPE, "coarse boxes overlap");
+  }
+  auto unit = [](const int c[3]) {
+    HBox b;
+    for (int d = 0; d < 3; ++d) { b.lo[d] = c[d]; b.hi[d] = c[d]; }
+    return b;
+  };
+  // ---- register faces, one pass over the faces of every coarsened fine box
+  for (int f = 0; f < nf; ++f)
+    for (int d = 0; d < dims; ++d)
+      for (int side = 0; side < 2; ++side) {
+        HBox plane = cf[f];
+        const int cd = side == 0 ? cf[f].lo[d] - 1 : cf[f].hi[d] + 1;  // cell across
+        plane.lo[d] = plane.hi[d] = cd;
+        for (int k = plane.lo[2]; k <= plane.hi[2]; ++k)
+          for (int j = plane.lo[1]; j <= plane.hi[1]; ++j)
+            for (int i = plane.lo[0]; i <= plane.hi[0]; ++i) {
+              int C[3] = {i, j, k};
+              if (C[d] < Dc.lo[d] || C[d] > Dc.hi[d]) {
+                if (!per[d]) continue;  // physical boundary: no register
+                C[d] += C[d] < Dc.lo[d] ? Lc[d] : -Lc[d];
+              }
+              FI.query(unit(C), hits);
+              if (!hits.empty()) continue;  // interior face of the fine region
+              CI.query(unit(C), hits);
+              if (hits.empty())
+                return rfail(R, MFM_ERR_UNSUPPORTED,
+                             "no coarse box next to the fine region (layouts not properly nested)");
+              RegFace e;
+              e.cb = hits[0];
+              e.d = d;
+              e.sigma = side == 0 ? +1 : -1;
+              e.fb = f;
+              const int t[3] = {i, j, k};
+              for (int q = 0; q < 3; ++q) {
+                e.c[q] = C[q];
+                e.f0[q] = q < dims ? t[q] * r : t[q];
+              }
+              e.f0[d] = side == 0 ? R->fine[f].lo[d] : R->fine[f].hi[d] + 1;
+              R->faces.push_back(e);
+            }
+      }
+  // group by coarse cell: box, cell (x fastest), then direction, low face first
+  std::sort(R->faces.begin(), R->faces.end(), [](const RegFace& a, const RegFace& b) {
+    if (a.cb != b.cb) return a.cb < b.cb;
+    for (int q = 2; q >= 0; --q)
+      if (a.c[q] != b.c[q]) return a.c[q] < b.c[q];
+    if (a.d != b.d) return a.d < b.d;
+    return a.sigma < b.sigma;
+  });
+  for (size_t e = 0; e < R->faces.size();) {
+    size_t e1 = e + 1;
+    while (e1 < R->faces.size() && R->faces[e1].cb == R->faces[e].cb &&
+           R->faces[e1].c[0] == R->faces[e].c[0] && R->faces[e1].c[1] == R->faces[e].c[1] &&
+           R->faces[e1].c[2] == R->faces[e].c[2])
+      ++e1;
+    R->cells.push_back(RegCell{(int)e, (int)e1});
+    e = e1;
+  }
+  // ---- average down: coarsened fine boxes onto the coarse boxes
+  for (int f = 0; f < nf; ++f) {
+    CI.query(cf[f], hits);
+    for (int cb : hits) {
+      const HBox I = inter(cf[f], R->crse[cb]);
+      if (empty(I)) continue;
+      AvgRect a;
+      a.fb = f;
+      a.cb = cb;
+      for (int d = 0; d < 3; ++d) { a.lo[d] = I.lo[d]; a.n[d] = (int)I.n(d); }
+      a.blk0 = R->avg_blk;
+      R->avg_blk += (int)((I.size() + kFB - 1) / kFB);
+      R->avg_cells += I.size();
+      R->avg.push_back(a);
+    }
+  }
+  return MFM_OK;
+}
+
+int reg_upload(mfm_ctx* c, mfm_flux_register* R) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return mfm::host::hipfail(c, e, "hipGetDevice");
+  if (R->dmem) {
+    if (dev != R->device) return mfm::host::fail(c, MFM_ERR_ARG, "flux register used on another device");
+    return MFM_OK;
+  }
+  using mfm::host::align256;
+  const size_t b_f = align256(R->faces.size() * sizeof(RegFace));
+  const size_t b_c = align256(R->cells.size() * sizeof(RegCell));
+  const size_t b_a = align256(R->avg.size() * sizeof(AvgRect));
+  const size_t b_r = align256(R->faces.size() * (size_t)R->nc * sizeof(double));
+  e = hipMalloc((void**)&R->dmem, b_f + b_c + b_a + b_r + 256);
+  if (e != hipSuccess) { R->dmem = nullptr; return mfm::host::fail(c, MFM_ERR_NOMEM, "hipMalloc of the flux register failed"); }
+  R->device = dev;
+  char* q = R->dmem;
+  R->d_faces = (RegFace*)q; q += b_f;
+  R->d_cells = (RegCell*)q; q += b_c;
+  R->d_avg = (AvgRect*)q; q += b_a;
+  R->d_reg = (double*)q;
+  struct Up { void* dst; const void* src; size_t bytes; } ups[] = {
+      {R->d_faces, R->faces.data(), R->faces.size() * sizeof(RegFace)},
+      {R->d_cells, R->cells.data(), R->cells.size() * sizeof(RegCell)},
+      {R->d_avg, R->avg.data(), R->avg.size() * sizeof(AvgRect)}};
+  for (const Up& u : ups) {
+    if (!u.bytes) continue;
+    e = hipMemcpy(u.dst, u.src, u.bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return mfm::host::hipfail(c, e, "flux register upload");
+  }
+  e = mfm::launch_reg_zero(R->d_reg, (long)R->faces.size() * R->nc, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return mfm::host::hipfail(c, e, "flux register clear");
+  return MFM_OK;
+}
+
+// Stage n FABs (each checked to hold `need[i]`) into table slot k.
+int reg_stage(mfm_ctx* c, mfm_flux_register* R, int k, const mfm_fab* F,
+              const std::vector<HBox>& need, const char* what, hipStream_t s) {
+  std::vector<FillFab> t(need.size());
+  for (size_t i = 0; i < need.size(); ++i) {
+    int rc = to_fillfab(c, F[i], R->nc, need[i], R->dims, t[i], what);
+    if (rc) return rc;
+  }
+  return slot_put(c, R->tab[k], t, s);
+}
+
+std::vector<HBox> face_boxes(const std::vector<HBox>& boxes, int dims) {
+  std::vector<HBox> v;
+  for (const HBox& b : boxes)
+    for (int d = 0; d < dims; ++d) {
+      HBox f = b;
+      f.hi[d] += 1;  // surroundingNodes(d)
+      v.push_back(f);
+    }
+  return v;
+}
+
+int reg_check(mfm_ctx* c, mfm_flux_register* R) {
+  if (!c || !R) return MFM_ERR_ARG;
+  if (!R->ok) return mfm::host::fail(c, MFM_ERR_ARG, "flux register was not built");
+  if (R->nc != c->C.nc) return mfm::host::fail(c, MFM_ERR_SHAPE, "flux register and context differ in ncomp");
+  return reg_upload(c, R);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mfm_flux_register_create(const mfm_params* p, const mfm_box* fine, int nf,
+                             const mfm_box* coarse, int ncb, int ref_ratio,
+                             mfm_flux_register** out) {
+  if (!out) return MFM_ERR_ARG;
+  *out = nullptr;
+  if (!p || nf < 0 || ncb < 0 || (nf > 0 && !fine) || (ncb > 0 && !coarse)) return MFM_ERR_ARG;
+  mfm_flux_register* R = new mfm_flux_register();
+  *out = R;
+  mfm::Consts C;
+  std::string why;
+  int rc = mfm::host::build_consts(p, C, why);
+  if (rc) return rfail(R, rc, why);
+  if (ref_ratio < 2) return rfail(R, MFM_ERR_ARG, "ref_ratio must be >= 2");
+  R->dims = p->spacedim;
+  R->nc = C.nc;
+  R->r = ref_ratio;
+  rc = build_register(R, p, fine, nf, coarse, ncb);
+  R->ok = rc == MFM_OK;
+  return rc;
+}
+
+int mfm_flux_register_destroy(mfm_flux_register* R) {
+  if (!R) return MFM_OK;
+  if (R->dmem) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(R->dmem);
+  }
+  for (TableSlot& t : R->tab) t.release();
+  delete R;
+  return MFM_OK;
+}
+
+const char* mfm_flux_register_last_error(const mfm_flux_register* R) {
+  return R ? R->err.c_str() : "null flux register";
+}
+
+int mfm_flux_register_info(const mfm_flux_register* R, long* faces, long* cells,
+                           long* average_cells) {
+  if (!R) return MFM_ERR_ARG;
+  if (faces) *faces = (long)R->faces.size();
+  if (cells) *cells = (long)R->cells.size();
+  if (average_cells) *average_cells = R->avg_cells;
+  return MFM_OK;
+}
+
+int mfm_flux_register_set_zero(mfm_ctx* c, mfm_flux_register* R, void* stream) {
+  int rc = reg_check(c, R);
+  if (rc) return rc;
+  hipError_t e = mfm::launch_reg_zero(R->d_reg, (long)R->faces.size() * R->nc, (hipStream_t)stream);
+  if (e != hipSuccess) return mfm::host::hipfail(c, e, "flux register clear");
+  return MFM_OK;
+}
+
+int mfm_flux_register_increment_coarse(mfm_ctx* c, mfm_flux_register* R, const mfm_fab* F,
+                                       double scale, void* stream) {
+  int rc = reg_check(c, R);
+  if (rc) return rc;
+  if (!F && !R->crse.empty()) return mfm::host::fail(c, MFM_ERR_ARG, "null flux table");
+  hipStream_t s = (hipStream_t)stream;
+  rc = reg_stage(c, R, 0, F, face_boxes(R->crse, R->dims), "coarse flux", s);
+  if (rc) return rc;
+  hipError_t e = mfm::launch_reg_coarse(R->d_faces, (long)R->faces.size(), R->tab[0].d, R->dims,
+                                        R->d_reg, R->nc, scale, s);
+  if (e != hipSuccess) return mfm::host::hipfail(c, e, "flux register increment");
+  return MFM_OK;
+}
+
+int mfm_flux_register_increment_fine(mfm_ctx* c, mfm_flux_register* R, const mfm_fab* F,
+                                     double scale, void* stream) {
+  int rc = reg_check(c, R);
+  if (rc) return rc;
+  if (!F && !R->fine.empty()) return mfm::host::fail(c, MFM_ERR_ARG, "null flux table");
+  hipStream_t s = (hipStream_t)stream;
+  rc = reg_stage(c, R, 1, F, face_boxes(R->fine, R->dims), "fine flux", s);
+  if (rc) return rc;
+  hipError_t e = mfm::launch_reg_fine(R->d_faces, (long)R->faces.size(), R->tab[1].d, R->dims, R->r,
+                                      R->d_reg, R->nc, scale, s);
+  if (e != hipSuccess) return mfm::host::hipfail(c, e, "flux register increment");
+  return MFM_OK;
+}
+
+int mfm_flux_register_reflux(mfm_ctx* c, mfm_flux_register* R, const mfm_fab* U,
+                             double scale, double dx_coarse, void* stream) {
+  int rc = reg_check(c, R);
+  if (rc) return rc;
+  if (!U && !R->crse.empty()) return mfm::host::fail(c, MFM_ERR_ARG, "null state table");
+  if (!(dx_coarse > 0.0)) return mfm::host::fail(c, MFM_ERR_ARG, "dx_coarse must be > 0");
+  hipStream_t s = (hipStream_t)stream;
+  rc = reg_stage(c, R, 2, U, R->crse, "coarse U", s);
+  if (rc) return rc;
+  // the inverse cell volume of the reference's Cartesian scalingFactor
+  // (src/CSHCartesian.cpp:290-305) is 1/dx
+  hipError_t e = mfm::launch_reflux(R->d_cells, (long)R->cells.size(), R->d_faces,
+                                    (long)R->faces.size(), R->tab[2].d, R->d_reg, R->nc, scale,
+                                    1.0 / dx_coarse, s);
+  if (e != hipSuccess) return mfm::host::hipfail(c, e, "reflux launch");
+  return MFM_OK;
+}
+
+int mfm_level_average_down_dev(mfm_ctx* c, mfm_flux_register* R, const mfm_fab* Uc,
+                               const mfm_fab* Uf, double dx_fine, void* stream) {
+  int rc = reg_check(c, R);
+  if (rc) return rc;
+  if ((!Uc && !R->crse.empty()) || (!Uf && !R->fine.empty()))
+    return mfm::host::fail(c, MFM_ERR_ARG, "null state table");
+  if (!(dx_fine > 0.0)) return mfm::host::fail(c, MFM_ERR_ARG, "dx_fine must be > 0");
+  hipStream_t s = (hipStream_t)stream;
+  std::vector<mfm_fab> both(Uc, Uc + R->crse.size());
+  both.insert(both.end(), Uf, Uf + R->fine.size());
+  std::vector<HBox> need = R->crse;
+  need.insert(need.end(), R->fine.begin(), R->fine.end());
+  rc = reg_stage(c, R, 3, both.data(), need, "U", s);
+  if (rc) return rc;
+  // getCellVolumes of the fine level (src/CSHCartesian.cpp:90-97)
+  double vol = dx_fine;
+  for (int d = 1; d < R->dims; ++d) vol = vol * dx_fine;
+  hipError_t e = mfm::launch_average_down(R->d_avg, (int)R->avg.size(), R->avg_blk, R->tab[3].d,
+                                          R->tab[3].d + R->crse.size(), R->nc, R->dims, R->r, vol, s);
+  if (e != hipSuccess) return mfm::host::hipfail(c, e, "average-down launch");
   return MFM_OK;
 }
 

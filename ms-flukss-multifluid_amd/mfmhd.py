@@ -103,6 +103,11 @@ EXPORTS = [
     "mfm_server_request_error", "mfm_fill_plan_create",
     "mfm_fill_plan_destroy", "mfm_fill_plan_last_error", "mfm_fill_plan_info",
     "mfm_level_exchange_dev", "mfm_level_fill_interp_dev", "mfm_fill_time_coef",
+    "mfm_flux_register_create", "mfm_flux_register_destroy",
+    "mfm_flux_register_last_error", "mfm_flux_register_info",
+    "mfm_flux_register_set_zero", "mfm_flux_register_increment_coarse",
+    "mfm_flux_register_increment_fine", "mfm_flux_register_reflux",
+    "mfm_level_average_down_dev",
 ]
 
 REQ_PENDING, REQ_ACTIVE, REQ_DONE, REQ_FAILED = range(4)
@@ -194,6 +199,16 @@ def load(path=LIB_PATH):
     lib.mfm_level_exchange_dev.argtypes = [vp, vp, FP, vp]
     lib.mfm_level_fill_interp_dev.argtypes = [vp, vp, FP, FP, FP, dbl, vp]
     lib.mfm_fill_time_coef.argtypes = [dbl, dbl, dbl, dbl, it, C.POINTER(dbl)]
+    lib.mfm_flux_register_create.argtypes = [P, BP, it, BP, it, it, C.POINTER(vp)]
+    lib.mfm_flux_register_destroy.argtypes = [vp]
+    lib.mfm_flux_register_last_error.argtypes = [vp]
+    lib.mfm_flux_register_last_error.restype = C.c_char_p
+    lib.mfm_flux_register_info.argtypes = [vp, C.POINTER(lng), C.POINTER(lng), C.POINTER(lng)]
+    lib.mfm_flux_register_set_zero.argtypes = [vp, vp, vp]
+    lib.mfm_flux_register_increment_coarse.argtypes = [vp, vp, FP, dbl, vp]
+    lib.mfm_flux_register_increment_fine.argtypes = [vp, vp, FP, dbl, vp]
+    lib.mfm_flux_register_reflux.argtypes = [vp, vp, FP, dbl, dbl, vp]
+    lib.mfm_level_average_down_dev.argtypes = [vp, vp, FP, FP, dbl, vp]
     lib.mfm_dev_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
     lib.mfm_dev_free.argtypes = [C.c_void_p]
     lib.mfm_dev_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
@@ -260,6 +275,14 @@ def num_ghost_cells(p):
 
 def fab(data_ptr, lo, hi, ncomp):
     return Fab(C.c_void_p(int(data_ptr)), Box.of(lo, hi), int(ncomp))
+
+
+def fab_array(fabs):
+    """A ctypes array of Fab from a list (an array passes through, so a
+    caller can build a level's table once)."""
+    if isinstance(fabs, C.Array):
+        return fabs
+    return (Fab * max(1, len(fabs)))(*fabs)
 
 
 class MfmError(RuntimeError):
@@ -359,13 +382,19 @@ class Context:
         return rc, nd.value, tuple(cell)
 
     def update_level(self, patches, dt, dx, time=0.0, stream=None):
-        """One launch sequence over a patch table (list of (U, valid)) ->
-        (level_dt, patch index, per-patch dts, per-patch cells)."""
+        """One launch sequence over a patch table (list of (U, valid) or
+        (U, valid, F) with F the patch's face-flux FABs, one per direction)
+        -> (level_dt, patch index, per-patch dts, per-patch cells)."""
         n = len(patches)
         tab = (PatchRec * n)()
-        for q, (U, valid) in enumerate(patches):
-            tab[q].U = U
-            tab[q].valid = valid
+        keep = []
+        for q, rec in enumerate(patches):
+            tab[q].U = rec[0]
+            tab[q].valid = rec[1]
+            if len(rec) > 2 and rec[2] is not None:
+                arr = (Fab * len(rec[2]))(*rec[2])
+                keep.append(arr)
+                tab[q].F = C.cast(arr, C.POINTER(Fab))
         nd = (C.c_double * n)()
         mc = (C.c_int * (3 * n))()
         ld = C.c_double()
@@ -395,7 +424,7 @@ class Context:
 
     def exchange(self, plan, fabs, stream=None):
         """LevelData::exchange of the level's FABs (in plan order)."""
-        arr = (Fab * len(fabs))(*fabs)
+        arr = fab_array(fabs)
         self._check(self.lib.mfm_level_exchange_dev(
             self.h, plan.h, arr, C.c_void_p(stream) if stream else None))
 
@@ -403,9 +432,9 @@ class Context:
         """PiecewiseLinearFillPatchMHDAM::fillInterp of the level's ghost
         cells from the coarser level (old and/or new data, either may be
         None)."""
-        arr = (Fab * len(fabs))(*fabs)
-        co = (Fab * len(coarse_old))(*coarse_old) if coarse_old is not None else None
-        cn = (Fab * len(coarse_new))(*coarse_new) if coarse_new is not None else None
+        arr = fab_array(fabs)
+        co = fab_array(coarse_old) if coarse_old is not None else None
+        cn = fab_array(coarse_new) if coarse_new is not None else None
         self._check(self.lib.mfm_level_fill_interp_dev(
             self.h, plan.h, arr, co, cn, float(alpha),
             C.c_void_p(stream) if stream else None))
@@ -442,6 +471,68 @@ class FillPlan:
     def close(self):
         if self.h:
             self.lib.mfm_fill_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class FluxRegister:
+    """Coarse-fine synchronisation of a level pair (mfm_flux_register_*):
+    the flux register of the interface and the average down.  Methods take
+    the Context whose stream and error slot they use."""
+
+    def __init__(self, params, fine_boxes, coarse_boxes, ref_ratio=2):
+        self.lib = load()
+        fb = (Box * max(1, len(fine_boxes)))(*[Box.of(lo, hi) for lo, hi in fine_boxes])
+        cb = (Box * max(1, len(coarse_boxes)))(*[Box.of(lo, hi) for lo, hi in coarse_boxes])
+        h = C.c_void_p()
+        rc = self.lib.mfm_flux_register_create(C.byref(params), fb, len(fine_boxes), cb,
+                                               len(coarse_boxes), int(ref_ratio), C.byref(h))
+        self.h = h
+        if rc != OK:
+            msg = self.lib.mfm_flux_register_last_error(h) if h else b""
+            self.close()
+            raise MfmError(rc, msg.decode() if msg else "")
+
+    def info(self):
+        f, c, a = C.c_long(), C.c_long(), C.c_long()
+        self.lib.mfm_flux_register_info(self.h, C.byref(f), C.byref(c), C.byref(a))
+        return {"faces": f.value, "cells": c.value, "average_cells": a.value}
+
+    @staticmethod
+    def _arr(fabs):
+        return fab_array(fabs)
+
+    def set_zero(self, ctx, stream=None):
+        ctx._check(self.lib.mfm_flux_register_set_zero(ctx.h, self.h,
+                                                       C.c_void_p(stream) if stream else None))
+
+    def increment_coarse(self, ctx, fluxes, scale, stream=None):
+        """fluxes: per coarse box, per direction (flattened box-major)."""
+        ctx._check(self.lib.mfm_flux_register_increment_coarse(
+            ctx.h, self.h, self._arr(fluxes), float(scale), C.c_void_p(stream) if stream else None))
+
+    def increment_fine(self, ctx, fluxes, scale, stream=None):
+        ctx._check(self.lib.mfm_flux_register_increment_fine(
+            ctx.h, self.h, self._arr(fluxes), float(scale), C.c_void_p(stream) if stream else None))
+
+    def reflux(self, ctx, U_coarse, scale, dx_coarse, stream=None):
+        ctx._check(self.lib.mfm_flux_register_reflux(
+            ctx.h, self.h, self._arr(U_coarse), float(scale), float(dx_coarse),
+            C.c_void_p(stream) if stream else None))
+
+    def average_down(self, ctx, U_coarse, U_fine, dx_fine, stream=None):
+        ctx._check(self.lib.mfm_level_average_down_dev(
+            ctx.h, self.h, self._arr(U_coarse), self._arr(U_fine), float(dx_fine),
+            C.c_void_p(stream) if stream else None))
+
+    def close(self):
+        if self.h:
+            self.lib.mfm_flux_register_destroy(self.h)
             self.h = None
 
     def __del__(self):

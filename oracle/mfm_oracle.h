@@ -48,6 +48,20 @@ int oracle_fill_interp(const mfm_params* p, const mfm_box* boxes, int n, mfm_fab
                        const mfm_fab* cnew, int r, int ng, double dxc, double alpha);
 /* interp_divB_0 on one slope triple (dBx/dx, dBy/dy, dBz/dz), in place */
 void oracle_interp_divb(double s[3], int dims);
+/* Coarse-fine synchronisation.  R: dense register over the coarse problem
+ * domain, R[(((d*2 + h) * ncell) + cell) * nc + comp] for the low (h = 0) or
+ * high (h = 1) face of a coarse cell in direction d; F: per box, per
+ * direction face FABs (box-major). */
+int oracle_fr_increment_coarse(const mfm_params* p, const mfm_box* fine, int nf,
+                               const mfm_box* crse, int ncb, int r, const mfm_fab* F,
+                               double scale, double* R);
+int oracle_fr_increment_fine(const mfm_params* p, const mfm_box* fine, int nf,
+                             const mfm_box* crse, int ncb, int r, const mfm_fab* F,
+                             double scale, double* R);
+int oracle_fr_reflux(const mfm_params* p, const mfm_box* fine, int nf, const mfm_box* crse,
+                     int ncb, int r, mfm_fab* U, double scale, double dxc, const double* R);
+int oracle_average_down(const mfm_params* p, const mfm_box* fine, int nf, const mfm_box* crse,
+                        int ncb, int r, mfm_fab* Uc, const mfm_fab* Uf, double dxf);
 
 /* ---- pointwise kernels, for the pinning tests ------------------------- */
 /* Primitive -> face flux.  wl/wr: plasma primitive states in the face's

@@ -7,7 +7,15 @@
 //   - PiecewiseLinearFillPatchMHDAM::define + fillInterp
 //     (srcIdeal/PiecewiseLinearFillPatchMHDAM.cpp:118-880) with interplimit_v
 //     and interp_divB_0 (srcIdeal/FineInterpMHDAMF.ChF:235-555), Cartesian
-//     coordinates (src/CSHCartesian.cpp: constant dx, PLFP_* no-ops).
+//     coordinates (src/CSHCartesian.cpp: constant dx, PLFP_* no-ops);
+//   - the coarse-fine synchronisation after the updates
+//     (srcIdeal/AMRLevelIdealMHD.cpp:695, 1282-1307, 1636-1647): the flux
+//     register (Chombo 3.2.4 LevelFluxRegister, not in the reference tree;
+//     restated from its published algorithm: the register holds, per coarse
+//     face of the coarse-fine interface, the time-integrated fine flux
+//     averaged over the face minus the coarse one, and reflux corrects the
+//     coarse cell outside the fine region by it) and the average down
+//     (CoarseAverageExt, src/CoarseAverageExt.cpp:126-194).
 //
 // TEST INFRASTRUCTURE ONLY, like mfm_oracle.cpp.  The IntVectSets of the
 // reference are kept as ordered sets of cells and built by the same set
@@ -469,6 +477,172 @@ int oracle_fill_interp(const mfm_params* p, const mfm_box* boxes, int n, mfm_fab
         }
       }
     }
+  return MFM_OK;
+}
+
+}  // extern "C"
+
+// ---- coarse-fine synchronisation ---------------------------------------------
+// The register is a dense array over the coarse problem domain:
+//   R[(((d * 2 + h) * ncell) + cell) * nc + comp]
+// for the face of side h (0 low, 1 high) of coarse cell `cell` in direction d.
+namespace {
+
+struct Sync {
+  PD FD, CD;
+  int dims, r;
+  std::vector<OBox> fb, cfb, cb;
+  long ncell;
+  Sync(const mfm_params* p, const mfm_box* fine, int nf, const mfm_box* crse, int ncb, int r_)
+      : dims(p->spacedim), r(r_) {
+    FD = level_domain(p);
+    CD = FD;
+    CD.dom = coarsenB(FD.dom, r, dims);
+    for (int i = 0; i < nf; ++i) { fb.push_back(ob(fine[i])); cfb.push_back(coarsenB(fb.back(), r, dims)); }
+    for (int j = 0; j < ncb; ++j) cb.push_back(ob(crse[j]));
+    ncell = CD.dom.n(0) * CD.dom.n(1) * CD.dom.n(2);
+  }
+  long cell(const IV& c) const {
+    return (c[0] - CD.dom.lo[0]) + CD.dom.n(0) * ((long)(c[1] - CD.dom.lo[1]) + CD.dom.n(1) * (c[2] - CD.dom.lo[2]));
+  }
+  // neighbour across face (d, h), wrapped; false outside a non-periodic domain
+  bool across(IV c, int d, int h, IV& n) const {
+    c[d] += h ? 1 : -1;
+    if (c[d] < CD.dom.lo[d] || c[d] > CD.dom.hi[d]) {
+      if (!CD.per[d]) return false;
+      c[d] += c[d] < CD.dom.lo[d] ? (int)CD.dom.n(d) : -(int)CD.dom.n(d);
+    }
+    n = c;
+    return true;
+  }
+  int fine_box_over(const IV& c) const {
+    for (size_t i = 0; i < cfb.size(); ++i)
+      if (cfb[i].has(c)) return (int)i;
+    return -1;
+  }
+  // (C, d, h) is a register face: C outside the fine region, the cell across
+  // inside it
+  bool is_face(const IV& c, int d, int h, int* f) const {
+    if (fine_box_over(c) >= 0) return false;
+    IV n;
+    if (!across(c, d, h, n)) return false;
+    const int i = fine_box_over(n);
+    if (i < 0) return false;
+    if (f) *f = i;
+    return true;
+  }
+  template <class Fn>
+  void each_face(Fn fn) const {
+    for (size_t j = 0; j < cb.size(); ++j) {
+      const OBox& b = cb[j];
+      for (int k = b.lo[2]; k <= b.hi[2]; ++k)
+        for (int jj = b.lo[1]; jj <= b.hi[1]; ++jj)
+          for (int i = b.lo[0]; i <= b.hi[0]; ++i)
+            for (int d = 0; d < dims; ++d)
+              for (int h = 0; h < 2; ++h) {
+                const IV c = {i, jj, k};
+                int f = -1;
+                if (is_face(c, d, h, &f)) fn((int)j, c, d, h, f);
+              }
+    }
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int oracle_fr_increment_coarse(const mfm_params* p, const mfm_box* fine, int nf,
+                               const mfm_box* crse, int ncb, int r, const mfm_fab* F,
+                               double scale, double* R) {
+  const Sync S(p, fine, nf, crse, ncb, r);
+  const int nc = F[0].ncomp;
+  S.each_face([&](int j, const IV& c, int d, int h, int) {
+    HFab f = hf(F[j * S.dims + d]);
+    IV face = c;
+    face[d] += h;
+    for (int q = 0; q < nc; ++q) {
+      double& reg = R[((long)(d * 2 + h) * S.ncell + S.cell(c)) * nc + q];
+      reg -= scale * f.at(face, q);
+    }
+  });
+  return MFM_OK;
+}
+
+int oracle_fr_increment_fine(const mfm_params* p, const mfm_box* fine, int nf,
+                             const mfm_box* crse, int ncb, int r, const mfm_fab* F,
+                             double scale, double* R) {
+  const Sync S(p, fine, nf, crse, ncb, r);
+  const int nc = F[0].ncomp;
+  double denom = 1.0;
+  for (int q = 0; q < S.dims - 1; ++q) denom *= r;
+  S.each_face([&](int, const IV& c, int d, int h, int fi) {
+    IV n;
+    S.across(c, d, h, n);
+    HFab f = hf(F[fi * S.dims + d]);
+    int t[2] = {-1, -1}, nt = 0;
+    for (int e = 0; e < S.dims; ++e)
+      if (e != d) t[nt++] = e;
+    for (int q = 0; q < nc; ++q) {
+      double& reg = R[((long)(d * 2 + h) * S.ncell + S.cell(c)) * nc + q];
+      for (int q2 = 0; q2 < (nt > 1 ? r : 1); ++q2)
+        for (int q1 = 0; q1 < r; ++q1) {
+          IV face = n;
+          face[d] = h ? n[d] * r : (n[d] + 1) * r;   // the fine region's boundary face
+          face[t[0]] = n[t[0]] * r + q1;
+          if (nt > 1) face[t[1]] = n[t[1]] * r + q2;
+          reg += (scale / denom) * f.at(face, q);
+        }
+    }
+  });
+  return MFM_OK;
+}
+
+int oracle_fr_reflux(const mfm_params* p, const mfm_box* fine, int nf, const mfm_box* crse,
+                     int ncb, int r, mfm_fab* U, double scale, double dxc, const double* R) {
+  const Sync S(p, fine, nf, crse, ncb, r);
+  const int nc = U[0].ncomp;
+  const double invvol = 1.0 / dxc;
+  S.each_face([&](int j, const IV& c, int d, int h, int) {
+    HFab u = hf(U[j]);
+    const double sigma = h ? 1.0 : -1.0;
+    for (int q = 0; q < nc; ++q) {
+      const double reg = R[((long)(d * 2 + h) * S.ncell + S.cell(c)) * nc + q];
+      u.at(c, q) = u.at(c, q) + (scale * sigma * invvol) * reg;
+    }
+  });
+  return MFM_OK;
+}
+
+int oracle_average_down(const mfm_params* p, const mfm_box* fine, int nf, const mfm_box* crse,
+                        int ncb, int r, mfm_fab* Uc, const mfm_fab* Uf, double dxf) {
+  const Sync S(p, fine, nf, crse, ncb, r);
+  const int dims = S.dims;
+  double vol = dxf;
+  for (int d = 1; d < dims; ++d) vol = vol * dxf;
+  for (int i = 0; i < nf; ++i) {
+    HFab f = hf(Uf[i]);
+    for (int j = 0; j < ncb; ++j) {
+      const OBox I = andB(S.cfb[i], S.cb[j]);
+      if (I.empty()) continue;
+      HFab u = hf(Uc[j]);
+      for (int var = 0; var < u.nc; ++var)
+        for (int kc = I.lo[2]; kc <= I.hi[2]; ++kc)
+          for (int jc = I.lo[1]; jc <= I.hi[1]; ++jc)
+            for (int ic = I.lo[0]; ic <= I.hi[0]; ++ic) {
+              double coarseSum = 0.0, volc = 0.0;
+              const int rz = dims == 3 ? r : 1;
+              for (int kk = 0; kk < rz; ++kk)
+                for (int jj = 0; jj < r; ++jj)
+                  for (int ii = 0; ii < r; ++ii) {
+                    const IV fc = {ic * r + ii, jc * r + jj, dims == 3 ? kc * r + kk : kc};
+                    coarseSum = coarseSum + vol * f.at(fc, var);
+                    volc = volc + vol;
+                  }
+              u.at({ic, jc, kc}, var) = coarseSum / volc;
+            }
+    }
+  }
   return MFM_OK;
 }
 

@@ -68,6 +68,13 @@ def load():
     lib.oracle_fill_interp.argtypes = [P, BP, C.c_int, F, BP, C.c_int, F, F,
                                        C.c_int, C.c_int, C.c_double, C.c_double]
     lib.oracle_interp_divb.argtypes = [dp, C.c_int]
+    for nm in ("oracle_fr_increment_coarse", "oracle_fr_increment_fine"):
+        getattr(lib, nm).argtypes = [P, BP, C.c_int, BP, C.c_int, C.c_int, F,
+                                     C.c_double, dp]
+    lib.oracle_fr_reflux.argtypes = [P, BP, C.c_int, BP, C.c_int, C.c_int, F,
+                                     C.c_double, C.c_double, dp]
+    lib.oracle_average_down.argtypes = [P, BP, C.c_int, BP, C.c_int, C.c_int, F, F,
+                                        C.c_double]
     _lib = lib
     return lib
 
@@ -251,3 +258,41 @@ def interp_divb(s, dims=3):
     v = np.array(s, dtype=np.float64)
     load().oracle_interp_divb(_ptr(v), int(dims))
     return v
+
+
+class FluxRegister:
+    """The oracle's flux register of a level pair: a dense array over the
+    coarse problem domain (see oracle_fr_* in mfm_oracle.h)."""
+
+    def __init__(self, p, fine_boxes, coarse_boxes, r, nc):
+        self.p, self.fb, self.cb, self.r, self.nc = p, fine_boxes, coarse_boxes, r, nc
+        dims = p.spacedim
+        ncell = 1
+        for d in range(3):
+            n = p.domain_hi[d] - p.domain_lo[d] + 1
+            ncell *= n // r if d < dims else n
+        self.R = np.zeros(2 * dims * ncell * nc)
+
+    def _args(self):
+        return (C.byref(self.p), _boxes(self.fb), len(self.fb), _boxes(self.cb), len(self.cb),
+                int(self.r))
+
+    def increment_coarse(self, F, los, scale):
+        rc = load().oracle_fr_increment_coarse(*self._args(), _fabs(F, los), float(scale),
+                                               _ptr(self.R))
+        assert rc == 0
+
+    def increment_fine(self, F, los, scale):
+        rc = load().oracle_fr_increment_fine(*self._args(), _fabs(F, los), float(scale),
+                                             _ptr(self.R))
+        assert rc == 0
+
+    def reflux(self, U, los, scale, dxc):
+        rc = load().oracle_fr_reflux(*self._args(), _fabs(U, los), float(scale), float(dxc),
+                                     _ptr(self.R))
+        assert rc == 0
+
+    def average_down(self, Uc, clos, Uf, flos, dxf):
+        rc = load().oracle_average_down(*self._args(), _fabs(Uc, clos), _fabs(Uf, flos),
+                                        float(dxf))
+        assert rc == 0
