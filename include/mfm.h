@@ -91,7 +91,7 @@ typedef struct mfm_params {
   int limiter[8];           /* limiter_RHO..limiter_BZ; all other slopes   */
                             /* (neutrals, psi) use minmod                  */
   int check_positivity;     /* mhdam.check_positivity                       */
-  int time_approx;          /* MFM_TIME_FIRST_ORDER or MFM_TIME_RK2         */
+  int time_approx;          /* MFM_TIME_FIRST_ORDER, _HANCOCK or _RK2       */
   int divb_method;          /* mhdam.divB_method: 0 none, 1 faces, 2 CD     */
   int eight_wave;           /* mhdam.8wave_use                              */
   int coord_sys;            /* MFM_COORD_*                                  */
@@ -180,7 +180,8 @@ enum {
   MFM_KT_FLUX_PLASMA = 1,  /* interior faces, plasma Riemann flux        */
   MFM_KT_FLUX_NEUTRAL = 2, /* interior faces, neutral van Leer fluxes    */
   MFM_KT_FLUX_BC = 3,      /* faces on a physical boundary (fluxBC)      */
-  MFM_KT_UPDATE = 4        /* flux divergence, sources, floors, CFL      */
+  MFM_KT_UPDATE = 4,       /* flux divergence, sources, floors, CFL      */
+  MFM_KT_PREDICT = 5       /* Hancock predictor (cell-local half step)   */
 };
 int mfm_ctx_set_timing(mfm_ctx* ctx, int on);
 /* hipGraph replay (no counterpart in the reference): when on, the launch

@@ -111,8 +111,14 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
     return MFM_ERR_ARG;
   }
   if (p->dedner && p->eight_wave) { why = "divB_cleaning = 5 can not be used with 8wave_use = 1"; return MFM_ERR_ARG; }
-  if (p->time_approx != MFM_TIME_RK2 && p->time_approx != MFM_TIME_FIRST_ORDER) {
-    why = "time_approximation must be 0 (first order) or 2 (RK2)";
+  if (p->time_approx != MFM_TIME_RK2 && p->time_approx != MFM_TIME_FIRST_ORDER &&
+      p->time_approx != MFM_TIME_HANCOCK) {
+    why = "time_approximation must be 0 (first order), 1 (Hancock) or 2 (RK2)";
+    return MFM_ERR_UNSUPPORTED;
+  }
+  if (p->time_approx == MFM_TIME_HANCOCK && p->dedner) {
+    // FLUXESHANCOCK has no flux for psi (srcIdeal/PatchIdealMHDF.ChF:235-297)
+    why = "GLM cleaning is not run with the Hancock scheme";
     return MFM_ERR_UNSUPPORTED;
   }
   if (p->coord_sys != MFM_COORD_CARTESIAN) { why = "only Cartesian coordinates"; return MFM_ERR_UNSUPPORTED; }
@@ -188,7 +194,7 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
 // Geometry of one patch's scratch.
 struct PatchGeo {
   HBox ub, vb1, vb2, fb[3];
-  size_t off_up, off_w, off_f[3], off_bn[3];
+  size_t off_up, off_w, off_w2, off_f[3], off_bn[3];
 };
 
 // One batched update of n patches.  Each patch names a row of the
@@ -201,8 +207,9 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
   const Consts& C0 = Ct[0];
   const mfm_params& p = c->p;
   const int dims = C0.dims;
-  const int ng = p.time_approx == MFM_TIME_RK2 ? 4 : 2;
   const bool rk2 = p.time_approx == MFM_TIME_RK2;
+  const bool han = p.time_approx == MFM_TIME_HANCOCK;
+  const int ng = rk2 ? 4 : (han ? 3 : 2);  // PatchMHDAM::numGhostCells
   if (n <= 0) return fail(c, MFM_ERR_ARG, "empty patch table");
   for (int r = 1; r < nct; ++r)
     if (Ct[r].dims != C0.dims || Ct[r].nf != C0.nf || Ct[r].dedner != C0.dedner)
@@ -232,10 +239,13 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     PatchGeo& g = G[q];
     g.ub = ub;
     g.vb2 = clip(Cq, v);
-    g.vb1 = rk2 ? clip(Cq, grow(v, 2, dims)) : g.vb2;
+    // first-stage value box: RK2 box + 2, Hancock predictor box + 1
+    g.vb1 = rk2 ? clip(Cq, grow(v, 2, dims)) : (han ? clip(Cq, grow(v, 1, dims)) : g.vb2);
     const long ucells = ub.size();
     g.off_up = off; off += align256((size_t)ucells * C0.nc * 8);
     g.off_w = off; off += align256((size_t)ucells * C0.nc * fs);
+    g.off_w2 = off;
+    if (han) off += align256((size_t)ucells * C0.nc * fs);
     for (int d = 0; d < dims; ++d) {
       g.fb[d] = g.vb1;
       g.fb[d].hi[d] += 1;
@@ -283,8 +293,30 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
       st1_prim.add(q, grow(g.vb1, 2, dims));
       for (int d = 0; d < dims; ++d) add_faces(st1_flux[d], st1_fbc[d], q, g.fb[d], d);
       st1_upd.add(q, g.vb1);
+    } else if (han) {
+      // predictor: W on its stencil (value box + 2), one kernel on the cells
+      st1_prim.add(q, grow(g.vb1, 2, dims));
+      st1_upd.add(q, g.vb1);
     }
-    if (rk2) {
+    if (han) {
+      // corrector: the predicted W of the physical ghost cells next to the
+      // box (the other cells of grow(vb2, 1) lie in vb1)
+      HBox rest = grow(g.vb2, 1, dims);
+      for (int d = 0; d < dims; ++d) {
+        if (rest.lo[d] < g.vb1.lo[d]) {
+          HBox b = rest;
+          b.hi[d] = g.vb1.lo[d] - 1;
+          st2_prim.add(q, b);
+          rest.lo[d] = g.vb1.lo[d];
+        }
+        if (rest.hi[d] > g.vb1.hi[d]) {
+          HBox b = rest;
+          b.lo[d] = g.vb1.hi[d] + 1;
+          st2_prim.add(q, b);
+          rest.hi[d] = g.vb1.hi[d];
+        }
+      }
+    } else if (rk2) {
       // the first stage leaves W of U* on vb1; the second stage only needs
       // the physical ghost cells of grow(vb2, 2) outside it
       HBox rest = grow(g.vb2, 2, dims);
@@ -362,6 +394,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     D.U = P.U.data;
     D.Up = reinterpret_cast<double*>(A + g.off_up);
     D.W = A + g.off_w;
+    D.W2 = han ? A + g.off_w2 : nullptr;
     for (int d = 0; d < 3; ++d) {
       D.ulo[d] = g.ub.lo[d];
       D.un[d] = (int)g.ub.n(d);
@@ -438,7 +471,16 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
       return launch_stage<double>(C0, ct, dp, L, S, dtp, idp, st, c->timer);
     };
     hipError_t e2;
-    if (rk2) {
+    if (han) {
+      // predictorHancock with dt/2, then correctorHancock with dt
+      StepArgs S1{0.5, 0, 0, 0, 0, 0, 0, 0, c->walk_tile, c->walk_xcd, 1};
+      e2 = launch(stage(0), S1);
+      if (e2 != hipSuccess) return hipfail(c, e2, "predictor launch");
+      if (c->timer) n_stage1 = c->timer->n;
+      StepArgs S2{1.0, 1, 0, 0, 0, 2, 1, 0, c->walk_tile, c->walk_xcd, 2};
+      e2 = launch(stage(8), S2);
+      if (e2 != hipSuccess) return hipfail(c, e2, "corrector launch");
+    } else if (rk2) {
       StepArgs S1{1.0, 0, 0, 1, 0, 0, 0, 1, c->walk_tile, c->walk_xcd};
       e2 = launch(stage(0), S1);
       if (e2 != hipSuccess) return hipfail(c, e2, "stage 1 launch");
@@ -463,7 +505,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     // depends on the arena layout and the list sizes.
     std::vector<long long> key = {(long long)(uintptr_t)A, (long long)off_patch,
                                   (long long)off_ct, (long long)off_dtp, (long long)off_idp,
-                                  (long long)off_dto, (long long)off_ido, dims, rk2 ? 1 : 0,
+                                  (long long)off_dto, (long long)off_ido, dims, p.time_approx,
                                   p.precision, C0.nf, C0.dedner, c->walk_tile,
                                   c->walk_xcd};
     for (int l = 0; l < kLists; ++l) {
@@ -506,7 +548,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     for (int q = 0; q < T.n; ++q) {
       float ms = 0.f;
       if (hipEventElapsedTime(&ms, T.ev[2 * q], T.ev[2 * q + 1]) != hipSuccess) continue;
-      const int st = (rk2 && q < n_stage1) ? 0 : 1;
+      const int st = ((rk2 || han) && q < n_stage1) ? 0 : 1;
       c->kt_ms[st][T.kind[q]] += ms;
       c->kt_n[st][T.kind[q]] += 1;
       c->kt_items[st][T.kind[q]] += (double)T.items[q];

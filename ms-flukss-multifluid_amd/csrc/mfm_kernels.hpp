@@ -14,6 +14,7 @@ struct DevPatch {
   double* U;        // caller state (updated in place on the valid box)
   double* Up;       // first-stage state U* (scratch)
   void* W;          // primitive variables (scratch, flux precision)
+  void* W2;         // Hancock: primitives of the predicted state
   void* F[3];       // face fluxes per direction (scratch, flux precision)
   void* Bn[3];      // face-normal B per direction (scratch)
   double* Fo[3];    // optional time-averaged fluxes returned to the caller
@@ -63,11 +64,14 @@ struct StepArgs {
   int write_w;         // store W of the stage output (input of the next stage)
   int row_tile;        // work walk: (y, z) tile of rows (see cell_of)
   int xcd_order;       // work walk: 1 = renumber blocks per XCD
+  int hancock;         // 0: Euler / RK2 stage; 1: Hancock predictor;
+                       // 2: Hancock corrector
 };
 
 // Optional kernel timing: a pair of events brackets every launch on the
 // launch stream; the host reads them after the step has synchronized.
-enum KernelKind { KK_PRIM = 0, KK_FLUX_PLASMA, KK_FLUX_NEUTRAL, KK_FLUX_BC, KK_UPDATE, KK_COUNT };
+enum KernelKind { KK_PRIM = 0, KK_FLUX_PLASMA, KK_FLUX_NEUTRAL, KK_FLUX_BC, KK_UPDATE,
+                  KK_PREDICT, KK_COUNT };
 struct KernelTimer {
   static constexpr int kMax = 32;
   hipEvent_t ev[2 * kMax];

@@ -73,7 +73,9 @@ size_t request_bytes(const mfm_params& cls, const Consts& C, const HBox& fab, co
   size_t b = align256(cells * C.nc * 8);          // U
   b += align256(cells * C.nc * 8);                // U*
   b += align256(cells * C.nc * fs);               // W
-  HBox v1 = cls.time_approx == MFM_TIME_RK2 ? grow(valid, 2, dims) : valid;
+  if (cls.time_approx == MFM_TIME_HANCOCK) b += align256(cells * C.nc * fs);  // W of U_pred
+  HBox v1 = cls.time_approx == MFM_TIME_RK2 ? grow(valid, 2, dims)
+            : (cls.time_approx == MFM_TIME_HANCOCK ? grow(valid, 1, dims) : valid);
   for (int d = 0; d < dims; ++d) {
     HBox f = v1;
     f.hi[d] += 1;
@@ -197,7 +199,7 @@ int mfm_server_submit(mfm_server* s, const mfm_params* p, double dx, int nsteps,
   }
   r->p = *p;
   const int dims = r->C.dims;
-  const int ng = p->time_approx == MFM_TIME_RK2 ? 4 : 2;
+  const int ng = mfm_num_ghost_cells(p);
   for (int d = 0; d < 3; ++d) {
     r->valid.lo[d] = d < dims ? p->domain_lo[d] : 0;
     r->valid.hi[d] = d < dims ? p->domain_hi[d] : 0;

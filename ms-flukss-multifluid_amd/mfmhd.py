@@ -113,8 +113,9 @@ EXPORTS = [
 REQ_PENDING, REQ_ACTIVE, REQ_DONE, REQ_FAILED = range(4)
 
 # kernel kinds of mfm_ctx_get_timing
-KT_PRIM, KT_FLUX_PLASMA, KT_FLUX_NEUTRAL, KT_FLUX_BC, KT_UPDATE = range(5)
-KT_NAMES = ("k_prim", "k_flux_plasma", "k_flux_neutral", "k_flux_bc", "k_update")
+KT_PRIM, KT_FLUX_PLASMA, KT_FLUX_NEUTRAL, KT_FLUX_BC, KT_UPDATE, KT_PREDICT = range(6)
+KT_NAMES = ("k_prim", "k_flux_plasma", "k_flux_neutral", "k_flux_bc", "k_update",
+            "k_hancock_pred")
 
 _lib = None
 

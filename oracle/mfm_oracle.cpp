@@ -1801,6 +1801,289 @@ void euler_step(const Eq& E, Fab& U, Fab* Fout, Fab* divBout, double dt,
   }
 }
 
+// FLUXESHANCOCK (srcIdeal/PatchIdealMHDF.ChF:235-297): physical MHD flux of
+// one primitive state
+void phys_flux_mhd(const Eq& E, const double* w, int idir, double* out) {
+  Rot r = rot(0, idir);
+  double rho = std::max(E.smallr, w[WRHO]);
+  double u = w[r.n], v = w[r.t1], ww = w[r.t2];
+  double pp = std::max(E.smallp, w[WPRES]);
+  double bx = w[r.bn], by = w[r.bt1], bz = w[r.bt2];
+  double b = d_1_8PI * (bx * bx + by * by + bz * bz);
+  double ke = half * (u * u + v * v + ww * ww);
+  double bu = bx * u + by * v + bz * ww;
+  double e = pp * E.invhgamma + rho * ke;
+  pp = pp + b;
+  e = e + b;
+  out[URHO] = rho * u;
+  out[UMOMX + idir] = rho * u * u + pp - d_1_4PI * bx * bx;
+  out[UMOMX + (idir + 1) % 3] = rho * u * v - d_1_4PI * bx * by;
+  out[UMOMX + (idir + 2) % 3] = rho * u * ww - d_1_4PI * bx * bz;
+  out[UENG] = u * (e + pp) - d_1_4PI * bx * bu;
+  out[UBX + idir] = zero;
+  out[UBX + (idir + 1) % 3] = u * by - v * bx;
+  out[UBX + (idir + 2) % 3] = u * bz - ww * bx;
+}
+
+// FLUXESHANCOCK_E (srcIdeal/PatchEulerF.ChF:417-468): physical Euler flux
+// of the fluid at iRho (conserved) / iRhoP (primitive)
+void phys_flux_euler(const Eq& E, const double* w, int idir, int iRho, int iRhoP,
+                     double* out) {
+  const int iMOMX = iRho + 1, iENG = iRho + 4, iVELX = iRhoP + 1, iPRES = iRhoP + 4;
+  double rho = std::max(E.smallr, w[iRhoP]);
+  double u = w[iVELX + idir], v = w[iVELX + (idir + 1) % 3], ww = w[iVELX + (idir + 2) % 3];
+  double p = std::max(E.smallp, w[iPRES]);
+  double ke = half * (u * u + v * v + ww * ww);
+  double e = p * E.invhgamma + rho * ke;
+  out[iRho] = rho * u;
+  out[iMOMX + idir] = rho * u * u + p;
+  out[iMOMX + (idir + 1) % 3] = rho * u * v;
+  out[iMOMX + (idir + 2) % 3] = rho * u * ww;
+  out[iENG] = u * (e + p);
+}
+
+// PatchMHDMF::computeBn method 1 (srcIdeal/PatchMHDMF.cpp:509-621) on the
+// faces of `faces`, then the physical-boundary faces of the Bn FAB box
+// `bnbox` take the face value of the cell inside; method 2 COMPUTEBNCD.
+void compute_bn(const Eq& E, const Fab& W, const Fab& WM, const Fab& WP, Fab& Bn, int d,
+                const Box& faces, const Box& bnbox) {
+  int io = d == 0, jo = d == 1, ko = d == 2;
+  int meth = E.P->divb_method;
+  if (meth == 1) {
+    FORBOX(faces, i, j, k)
+      Bn(i, j, k, 0) = half * (WP(i - io, j - jo, k - ko, WBX + d) + WM(i, j, k, WBX + d));
+    if (E.physical(d)) {
+      if (bnbox.lo[d] == E.P->domain_lo[d]) {
+        Box s = bnbox; s.hi[d] = s.lo[d];
+        FORBOX(s, i, j, k) Bn(i, j, k, 0) = WM(i, j, k, WBX + d);
+      }
+      if (bnbox.hi[d] == E.P->domain_hi[d] + 1) {
+        Box s = bnbox; s.lo[d] = s.hi[d];
+        FORBOX(s, i, j, k) Bn(i, j, k, 0) = WP(i - io, j - jo, k - ko, WBX + d);
+      }
+    }
+  } else if (meth == 2) {
+    FORBOX(faces, i, j, k)
+      Bn(i, j, k, 0) = half * (W(i - io, j - jo, k - ko, WBX + d) + W(i, j, k, WBX + d));
+  }
+}
+
+// addExplicitSources (srcIdeal/PatchMHDMF.cpp:1149-1389) on valueBox with
+// the primitive state W, face Bn and regions REG (8-wave, PSIPARABOLIC,
+// charge exchange), as in euler_step.
+void explicit_sources(const Eq& E, Fab& U, const Fab& W, const Fab* Bn, IFab& REG,
+                      bool ce, double dt, double dx, const Box& valueBox, Fab* divBout) {
+  const int dims = E.dims;
+  Fab divB;
+  divB.define(valueBox, 1, 0.0);
+  for (int d = 0; d < dims; ++d) {  // COMPUTEDIVB :472-492
+    int io = d == 0, jo = d == 1, ko = d == 2;
+    FORBOX(valueBox, i, j, k) {
+      divB(i, j, k, 0) = divB(i, j, k, 0) + Bn[d](i + io, j + jo, k + ko, 0) -
+                         Bn[d](i, j, k, 0);
+    }
+  }
+  if (E.P->eight_wave) {  // SOURCE8WAVES :681-722 + ADDSOURCES UMOMX..UBZ
+    double dtbydx = dt / dx;
+    PFORBOX(valueBox, i, j, k) {
+      double dB = divB(i, j, k, 0);
+      dB = -dB * dtbydx;
+      double Ux = W(i, j, k, WVELX), Uy = W(i, j, k, WVELY), Uz = W(i, j, k, WVELZ);
+      double Bx = d_1_4PI * W(i, j, k, WBX), By = d_1_4PI * W(i, j, k, WBY),
+             Bz = d_1_4PI * W(i, j, k, WBZ);
+      double S[8];
+      S[UMOMX] = dB * Bx; S[UMOMY] = dB * By; S[UMOMZ] = dB * Bz;
+      S[UENG] = dB * (Bx * Ux + By * Uy + Bz * Uz);
+      S[UBX] = dB * Ux; S[UBY] = dB * Uy; S[UBZ] = dB * Uz;
+      for (int c = UMOMX; c <= UBZ; ++c) U(i, j, k, c) = U(i, j, k, c) + S[c];
+    }
+    if (divBout && divBout->p) {
+      FORBOX(valueBox, i, j, k) {
+        if (i >= divBout->b.lo[0] && i <= divBout->b.hi[0] &&
+            j >= divBout->b.lo[1] && j <= divBout->b.hi[1] &&
+            k >= divBout->b.lo[2] && k <= divBout->b.hi[2])
+          (*divBout)(i, j, k, 0) = divB(i, j, k, 0);
+      }
+    }
+  }
+  if (E.dedner) {  // PSIPARABOLIC (src/DednerF.ChF:65-87)
+    double coeff = std::exp(-dt * E.ch * E.ch / (E.cp * E.cp));
+    FORBOX(valueBox, i, j, k) U(i, j, k, E.iPsi) = coeff * U(i, j, k, E.iPsi);
+  }
+  if (ce) {
+    PFORBOX(valueBox, i, j, k) {
+      double w[32], s[32];
+      for (int c = 0; c < E.nc; ++c) w[c] = W(i, j, k, c);
+      charge_exchange_cell(E, w, REG(i, j, k), dt, s);
+      for (int c = UMOMX; c < E.nc; ++c) U(i, j, k, c) = U(i, j, k, c) + s[c];
+    }
+  }
+}
+
+// updateState with time_approximation = 1: predictorHancock on a copy of U
+// with dt/2, then correctorHancock on U with dt (src/PatchMHDAM.cpp:1288-1300,
+// 858-1189).  The predictor is cell-local in time: physical fluxes of the
+// cell's own face values (FLUXESHANCOCK), no Riemann problem; its state moves
+// the face values by W(U_pred) - W(U) (UPDATEVALUESONFACES), and the
+// corrector solves the Riemann problems on those.
+void hancock_step(const Eq& E, Fab& U, Fab* Fout, Fab* divBout, double dt, double dx,
+                  double time, const Box& box) {
+  const int dims = E.dims;
+  const Box UBox = U.b;
+  const Box WBoxB = E.clip(UBox);
+  const bool ce = (E.nf >= 2 && E.P->problem == MFM_PROBLEM_HELIO);
+  const double invvol = one / dx;
+  // ---------------------------------------------------------- predictor --
+  const double dtp = 0.5 * dt;
+  Fab UP;
+  UP.define(UBox, E.nc, 0.0);
+  std::memcpy(UP.p, U.p, sizeof(double) * (size_t)U.sc * E.nc);
+  Fab W;
+  W.define(UBox, E.nc, std::numeric_limits<double>::quiet_NaN());
+  cons_to_prim(E, W, UP, WBoxB);
+  for (int d = 0; d < dims; ++d) fill_ghost(E, W, d);
+  const Box valueP = E.clip(grow(box, 1, dims));
+  const Box slopeBox = grow(valueP, 1, dims);
+  IFab REG;
+  if (ce) {
+    REG.define(slopeBox);
+    FORBOX(slopeBox, i, j, k) {
+      double w[32];
+      for (int c = 0; c < E.nc; ++c) w[c] = W(i, j, k, c);
+      REG(i, j, k) = region_cell(E, w, i, j, k, dx);
+    }
+  }
+  Fab WM[3], WP[3], Bn[3], FM[3], FP[3];
+  Box fluxP[3];
+  for (int d = 0; d < dims; ++d) {
+    WM[d].define(slopeBox, E.nc, 0.0);
+    WP[d].define(slopeBox, E.nc, 0.0);
+    face_values(E, W, WM[d], WP[d], d, slopeBox);
+    FM[d].define(valueP, E.nc, 0.0);
+    FP[d].define(valueP, E.nc, 0.0);
+    PFORBOX(valueP, i, j, k) {  // fluxesHancock (srcIdeal/PatchMHDMF.cpp:307-406)
+      double wm[32], wp[32], fm[32], fp[32];
+      for (int c = 0; c < E.nc; ++c) { wm[c] = WM[d](i, j, k, c); wp[c] = WP[d](i, j, k, c); }
+      phys_flux_mhd(E, wm, d, fm);
+      phys_flux_mhd(E, wp, d, fp);
+      for (int f = 1; f < E.nf; ++f) {
+        const int s = E.iRhoN + (f - 1) * WNUM_E;
+        phys_flux_euler(E, wm, d, s, s, fm);
+        phys_flux_euler(E, wp, d, s, s, fp);
+      }
+      for (int c = 0; c < E.nc; ++c) { FM[d](i, j, k, c) = fm[c]; FP[d](i, j, k, c) = fp[c]; }
+    }
+    fluxP[d] = valueP;
+    fluxP[d].hi[d] += 1;
+    Bn[d].define(fluxP[d], 1, 0.0);
+    compute_bn(E, W, WM[d], WP[d], Bn[d], d, fluxP[d], fluxP[d]);
+  }
+  for (int d = 0; d < dims; ++d) {  // updateHancock -> UPDATECONSHANCOCK (areas = 1)
+    for (int c = 0; c < E.nc; ++c) {
+      PFORBOX(valueP, i, j, k) {
+        UP(i, j, k, c) = UP(i, j, k, c) -
+                         dtp * invvol * (one * FP[d](i, j, k, c) - one * FM[d](i, j, k, c));
+      }
+    }
+  }
+  explicit_sources(E, UP, W, Bn, REG, ce, dtp, dx, valueP, nullptr);
+  Fab WPred;
+  WPred.define(UBox, E.nc, std::numeric_limits<double>::quiet_NaN());
+  cons_to_prim(E, WPred, UP, WBoxB);
+  for (int d = 0; d < dims; ++d) fill_ghost(E, WPred, d);
+  for (int d = 0; d < dims; ++d) {  // UPDATEVALUESONFACES (src/PatchMHDAMF.ChF:640-661)
+    for (int c = 0; c < E.nc; ++c) {
+      FORBOX(slopeBox, i, j, k) {
+        double aux = WPred(i, j, k, c) - W(i, j, k, c);
+        WM[d](i, j, k, c) = WM[d](i, j, k, c) + aux;
+        WP[d](i, j, k, c) = WP[d](i, j, k, c) + aux;
+      }
+    }
+    compute_bn(E, W, WM[d], WP[d], Bn[d], d, fluxP[d], fluxP[d]);
+  }
+  // ---------------------------------------------------------- corrector --
+  const Box valueBox = E.clip(box);
+  IFab REGc;
+  if (ce) {
+    REGc.define(valueBox);
+    FORBOX(valueBox, i, j, k) {
+      double w[32];
+      for (int c = 0; c < E.nc; ++c) w[c] = WPred(i, j, k, c);
+      REGc(i, j, k) = region_cell(E, w, i, j, k, dx);
+    }
+  }
+  for (int d = 0; d < dims; ++d) {
+    int io = d == 0, jo = d == 1, ko = d == 2;
+    Box faceBox = grow_dir(valueBox, d, 1);
+    faceBox = E.clip(faceBox);
+    faceBox = grow_dir(faceBox, d, -1);
+    faceBox.hi[d] += 1;
+    Box fluxBox = valueBox;
+    fluxBox.hi[d] += 1;
+    Fout[d].define(fluxBox, E.nc, 0.0);
+    Fab& F = Fout[d];
+    FORBOX(fluxBox, i, j, k) F(i, j, k, WBX + d) = Bn[d](i, j, k, 0);
+    PFORBOX(faceBox, i, j, k) {
+      double wl[32], wr[32], out[32];
+      for (int c = 0; c < E.nc; ++c) {
+        wl[c] = WP[d](i - io, j - jo, k - ko, c);
+        wr[c] = WM[d](i, j, k, c);
+        out[c] = F(i, j, k, c);
+      }
+      plasma_flux_face(E, wl, wr, d, out);
+      for (int f = 1; f < E.nf; ++f)
+        neutral_flux_face(E, wl, wr, d, E.iRhoN + (f - 1) * WNUM_E, out);
+      for (int c = 0; c < E.nc; ++c) F(i, j, k, c) = out[c];
+    }
+    if (E.physical(d)) {
+      for (int side = 0; side < 2; ++side) {
+        bool touches = side == 0 ? valueBox.lo[d] == E.P->domain_lo[d]
+                                 : valueBox.hi[d] == E.P->domain_hi[d];
+        if (!touches) continue;
+        Box sb = fluxBox;
+        if (side == 0) sb.hi[d] = sb.lo[d]; else sb.lo[d] = sb.hi[d];
+        FORBOX(sb, i, j, k) {
+          double wl[32], wr[32], out[32], bn;
+          for (int c = 0; c < E.nc; ++c) {
+            wl[c] = WP[d](i - io, j - jo, k - ko, c);
+            wr[c] = WM[d](i, j, k, c);
+            out[c] = F(i, j, k, c);
+          }
+          bn = Bn[d](i, j, k, 0);
+          if (E.P->problem == MFM_PROBLEM_RIEMANN) {
+            riemann_flux_bc_face(E, side, out, &bn);
+          } else if (E.P->problem == MFM_PROBLEM_HELIO) {
+            plasma_flux_face(E, wl, wr, d, out);
+            for (int f = 1; f < E.nf; ++f)
+              neutral_flux_face(E, wl, wr, d, E.iRhoN + (f - 1) * WNUM_E, out);
+            helio_bc_face(E, side == 0 ? -1 : 1, d, dx, i, j, k, wl, wr, out, &bn);
+          }
+          for (int c = 0; c < E.nc; ++c) F(i, j, k, c) = out[c];
+          Bn[d](i, j, k, 0) = bn;
+        }
+      }
+    }
+  }
+  for (int d = 0; d < dims; ++d) {  // finalUpdate -> UPDATECONS
+    int io = d == 0, jo = d == 1, ko = d == 2;
+    Fab& F = Fout[d];
+    for (int c = 0; c < E.nc; ++c) {
+      PFORBOX(valueBox, i, j, k) {
+        U(i, j, k, c) = U(i, j, k, c) -
+                        dt * invvol * (F(i + io, j + jo, k + ko, c) - F(i, j, k, c));
+      }
+    }
+  }
+  explicit_sources(E, U, WPred, Bn, REGc, ce, dt, dx, valueBox, divBout);
+  if (E.P->problem == MFM_PROBLEM_HELIO) helio_reinit(E, U, dx, dt + time, valueBox);
+  PFORBOX(valueBox, i, j, k) {
+    double u[32];
+    for (int c = 0; c < E.nc; ++c) u[c] = U(i, j, k, c);
+    floors_cell(E, u);
+    for (int c = 0; c < E.nc; ++c) U(i, j, k, c) = u[c];
+  }
+}
+
 // PatchMHDMF::computeDt + PhysProblem/HeliosphericProblem::computeDt
 // (swlism/HeliosphericProblem.cpp:3809-3860); the exclusion radius is
 // m_R0dtIgnore = m_R0 - 2 dx of the level (:497).
@@ -1837,8 +2120,11 @@ int check_args(const mfm_params* p) {
   if (p->nfluids < 1 || p->nfluids > 4) return MFM_ERR_UNSUPPORTED;
   if (p->dedner && p->nfluids != 1) return MFM_ERR_UNSUPPORTED;
   if (p->spacedim != 2 && p->spacedim != 3) return MFM_ERR_ARG;
-  if (p->time_approx != MFM_TIME_RK2 && p->time_approx != MFM_TIME_FIRST_ORDER)
+  if (p->time_approx != MFM_TIME_RK2 && p->time_approx != MFM_TIME_FIRST_ORDER &&
+      p->time_approx != MFM_TIME_HANCOCK)
     return MFM_ERR_UNSUPPORTED;
+  // FLUXESHANCOCK has no psi flux: the GLM system is not run with Hancock
+  if (p->time_approx == MFM_TIME_HANCOCK && p->dedner) return MFM_ERR_UNSUPPORTED;
   if (p->coord_sys != MFM_COORD_CARTESIAN) return MFM_ERR_UNSUPPORTED;
   return MFM_OK;
 }
@@ -1883,7 +2169,8 @@ int oracle_patch_update(const mfm_params* p, double ch, double cp, mfm_fab Uf,
   Eq E(p, ch, cp);
   Fab U; U.wrap(Uf);
   Box box = mkbox(validb);
-  int ng = p->time_approx == MFM_TIME_RK2 ? 4 : 2;
+  // PatchMHDAM::numGhostCells (src/PatchMHDAM.cpp:280-286)
+  int ng = p->time_approx == MFM_TIME_RK2 ? 4 : (p->time_approx == MFM_TIME_HANCOCK ? 3 : 2);
   if (!contains(U.b, grow(box, ng, E.dims)) || U.nc != E.nc) return MFM_ERR_SHAPE;
   Fab dB; if (divB) dB.wrap(*divB);
   Fab Fo[3];
@@ -1916,6 +2203,8 @@ int oracle_patch_update(const mfm_params* p, double ch, double cp, mfm_fab Uf,
       if (!in)
         for (int c = 0; c < E.nc; ++c) U(i, j, k, c) = Uold(i, j, k, c);
     }
+  } else if (p->time_approx == MFM_TIME_HANCOCK) {
+    hancock_step(E, U, Fo, divB ? &dB : nullptr, dt, dx, time, box);
   } else {
     euler_step(E, U, Fo, divB ? &dB : nullptr, dt, dx, time, box);
   }

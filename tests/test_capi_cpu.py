@@ -109,7 +109,12 @@ def test_ctx_validation_without_device():
     p.eight_wave = 1
     assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_ARG
     p = mfmhd.default_params()
-    p.time_approx = mfmhd.TIME_HANCOCK
+    p.time_approx = mfmhd.TIME_HANCOCK   # Hancock without GLM cleaning only
+    p.dedner = 1
+    p.eight_wave = 0
+    assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
+    p = mfmhd.default_params()
+    p.time_approx = 3
     assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
     if mfmhd.device_count() == 0:
         p = mfmhd.default_params()
