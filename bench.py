@@ -370,6 +370,53 @@ def fill_bench(box=16, per_side=10, levels=3, reps=5):
                              "increments, reflux, average down"}}
 
 
+def hancock_bench(prec, n=256, steps=10, warmup=3, walk=None):
+    """The same 256^3 4-fluid heliospheric workload with the Hancock
+    predictor-corrector (time_approximation = 1, the scheme of the
+    reference's 3-D multi-fluid deck main/inputs/helio03.inputs): one Riemann
+    solve per step instead of two.  Same dt control."""
+    p, dx = c3_params(n, prec)
+    p.time_approx = mfmhd.TIME_HANCOCK
+    ctx = mfmhd.Context(p)
+    if walk:
+        ctx.set_walk(walk[0], walk[1])
+    nc, ng = ctx.ncomp, ctx.nghost
+    lo, hi = (-ng,) * 3, (n - 1 + ng,) * 3
+    m = n + 2 * ng
+    U = mfmhd.DeviceArray((nc, m, m, m))
+    Uf = mfmhd.fab(U.ptr, lo, hi, nc)
+    ctx.init_problem(Uf, mfmhd.Box.of(lo, hi), dx)
+    valid = mfmhd.Box.of((0, 0, 0), (n - 1,) * 3)
+    dtn, _ = ctx.compute_dt(Uf, valid, dx)
+    t, dt = 0.0, None
+    for _ in range(warmup):
+        dt = CFL0 * dtn if dt is None else min(CFL * dtn, DT_GROWTH * dt)
+        dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
+        t += dt
+    mfmhd.synchronize()
+    ctx.set_timing(True)
+    e0, e1 = mfmhd.Event(), mfmhd.Event()
+    e0.record()
+    for _ in range(steps):
+        dt = min(CFL * dtn, DT_GROWTH * dt)
+        dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
+        t += dt
+    e1.record()
+    mfmhd.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    kern = {}
+    for st, stage in ((0, "predictor"), (1, "corrector")):
+        for kind, name in enumerate(mfmhd.KT_NAMES):
+            kms, kn, kit = ctx.timing(st, kind)
+            if kn:
+                kern["%s/%s" % (name, stage)] = {"avg_ms": kms / kn, "launches": kn}
+    ctx.close()
+    U.free()
+    return {"scheme": "Hancock predictor-corrector (time_approximation = 1)",
+            "ghost": ng, "ms_per_step": ms, "cell_updates_per_s": float(n) ** 3 / (ms * 1e-3),
+            "kernels": kern}
+
+
 def copy_ceiling(gib=4, reps=5):
     """Device-to-device copy bandwidth (read + write bytes per second): the
     HBM rate a streaming kernel can reach on this device."""
@@ -407,6 +454,8 @@ def main():
                     help="keep the default work walk instead of autotuning it")
     ap.add_argument("--no-amr", action="store_true",
                     help="skip the batched AMR patch-table measurement")
+    ap.add_argument("--no-hancock", action="store_true",
+                    help="skip the Hancock-scheme measurement of the same workload")
     ap.add_argument("--no-fill", action="store_true",
                     help="skip the level ghost-fill measurement")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -529,6 +578,8 @@ def main():
         out["ensemble"] = serve_bench(prec, n_req=args.serve_requests)
     if not args.no_amr:
         out["amr_batched"] = amr_bench(prec)
+    if not args.no_hancock:
+        out["hancock"] = hancock_bench(prec, n, args.steps, args.warmup, walk)
     if not args.no_fill:
         out["ghost_fill"] = fill_bench()
     out["roofline"]["copy_ceiling_GBps"] = copy_ceiling() / 1e9
