@@ -97,3 +97,67 @@ def test_hancock_rejects_glm():
     with pytest.raises(mfmhd.MfmError) as e:
         mfmhd.Context(p)
     assert e.value.code == mfmhd.ERR_UNSUPPORTED
+
+
+def test_level_table_hancock(oracle_mod):
+    """mfm_level_update_dev with Hancock over uneven patches == one call
+    per patch, and a patch == the oracle."""
+    from helpers import gpu_update
+    p = periodic_params(nf=4, flux=mfmhd.FLUX_ROE8, lim=mfmhd.LIM_MINMOD,
+                        time=mfmhd.TIME_HANCOCK)
+    nc = mfmhd.num_components(p)
+    rng = np.random.default_rng(9)
+    states = []
+    for q in range(5):
+        e = int(rng.choice([4, 8, 12]))
+        lo0 = rng.integers(-20, 20, size=3)
+        vlo, vhi = tuple(int(v) for v in lo0), tuple(int(v) + e - 1 for v in lo0)
+        lo, hi, shape = grid_shape(nc, vlo, vhi, NG)
+        states.append((smooth_state(p, shape, lo, seed=30 + q), lo, vlo, vhi))
+    ctx = mfmhd.Context(p)
+    singles = [gpu_update(ctx, U, lo, vlo, vhi, 0.01, 0.05) for U, lo, vlo, vhi in states]
+    pats = [GpuPatch(U, lo) for U, lo, _, _ in states]
+    ctx.update_level([(g.fab, mfmhd.Box.of(s[2], s[3])) for g, s in zip(pats, states)],
+                     0.01, 0.05)
+    mfmhd.synchronize()
+    for q, (g, s) in enumerate(zip(pats, singles)):
+        assert np.array_equal(g.numpy(), s["U"]), q
+    U, lo, vlo, vhi = states[1]
+    o = oracle_mod.patch_update(p, U, lo, vlo, vhi, 0.01, 0.05)
+    check64(singles[1], o, lo, vlo, vhi)
+    ctx.close()
+
+
+def test_served_hancock_request_matches_oracle(oracle_mod):
+    """The serving loop with a Hancock equation class: a request run alone
+    equals the oracle stepped with the same CFL rule."""
+    from test_serve import req_params
+    n = 16
+    p, dx = req_params(n)
+    p.time_approx = mfmhd.TIME_HANCOCK
+    srv = mfmhd.Server(p, 8 << 30)
+    rid = srv.submit(p, dx, 3, 0.5)
+    for _ in range(50):
+        na, npd, _ = srv.step()
+        if srv.info(rid)["state"] >= mfmhd.REQ_DONE:
+            break
+    inf = srv.info(rid)
+    assert inf["state"] == mfmhd.REQ_DONE, srv.error(rid)
+    Ug = srv.fetch(rid)
+    srv.close()
+    lo, hi = inf["lo"], inf["hi"]
+    assert lo == (-NG, -NG, -NG)
+    vlo, vhi = (0, 0, 0), (n - 1, n - 1, n - 1)
+    shape = (mfmhd.num_components(p), hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+    U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
+    dtn, _ = oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)
+    t = 0.0
+    for _ in range(3):
+        dt = 0.5 * dtn
+        r = oracle_mod.patch_update(p, U, lo, vlo, vhi, dt, dx, time=t)
+        assert r["rc"] == 0
+        U, dtn = r["U"], r["dt"]
+        t += dt
+    from helpers import rel_err_per_comp
+    err = rel_err_per_comp(valid_view(Ug, lo, vlo, vhi), valid_view(U, lo, vlo, vhi)).max()
+    assert err <= 1e-12, err
