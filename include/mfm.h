@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define MFM_ABI_VERSION 1
+#define MFM_ABI_VERSION 2
 
 /* ---- return codes ---------------------------------------------------- */
 enum {
@@ -128,6 +128,10 @@ typedef struct mfm_params {
   double ce_sigma_lindsay;        /* 1e-16 * scaleLen * lismN             */
   int    ce_single_literals;      /* 1: Fortran default-real literals are */
                                   /* REAL*4 (13.493, 0.531, ...)          */
+  /* mhdam.char_reconstruction (main/MHDAM.cpp:929-941): 1 = limit the    */
+  /* plasma slopes in characteristic variables (Reconstruction::faceValues,*/
+  /* src/Reconstruction.cpp:178-204)                                     */
+  int    char_reconstruction;
 } mfm_params;
 
 /* sizeof(mfm_params), for FFI bindings to check their struct layout. */

@@ -122,6 +122,16 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
     return MFM_ERR_UNSUPPORTED;
   }
   if (p->coord_sys != MFM_COORD_CARTESIAN) { why = "only Cartesian coordinates"; return MFM_ERR_UNSUPPORTED; }
+  if (p->char_reconstruction && p->precision != MFM_PREC_FP64) {
+    why = "char_reconstruction runs in the FP64 build only";
+    return MFM_ERR_UNSUPPORTED;
+  }
+  if (p->char_reconstruction)
+    for (int v = 0; v < 8; ++v)
+      if (p->limiter[v] == MFM_LIM_NONE) {
+        why = "char_reconstruction needs a slope limiter on every plasma variable";
+        return MFM_ERR_UNSUPPORTED;
+      }
   if (p->precision != MFM_PREC_FP64 && p->precision != MFM_PREC_FP32_FLUX) { why = "bad precision"; return MFM_ERR_ARG; }
   if (p->divb_method < 0 || p->divb_method > 2) { why = "bad divB_method"; return MFM_ERR_ARG; }
   for (int v = 0; v < 8; ++v)
@@ -149,6 +159,7 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
   C.divb_method = p->divb_method;
   C.problem = p->problem;
   for (int c = 0; c < kMaxComp; ++c) C.lim[c] = (int8_t)(c < 8 ? p->limiter[c] : MFM_LIM_MINMOD);
+  C.char_rec = p->char_reconstruction ? 1 : 0;
   C.plasma_lim = C.lim[0];
   for (int c = 1; c < 8 + C.dedner; ++c)
     if (C.lim[c] != C.lim[0]) C.plasma_lim = -1;

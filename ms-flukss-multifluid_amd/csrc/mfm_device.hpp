@@ -47,6 +47,7 @@ struct Consts {
   int eight_wave, divb_method, problem;
   int8_t lim[kMaxComp];
   int plasma_lim;     // limiter shared by every plasma variable (psi incl.), or -1
+  int char_rec;       // plasma slopes limited in characteristic variables
   double gamma, hgamma, invhgamma, smallr, smallp, del2, smallB;
   double ch, cp;
   int dom_lo[3], dom_hi[3], phys[3];  // phys[d]: non-periodic physical dir
@@ -193,6 +194,230 @@ __device__ __forceinline__ void limit(int lim, T wl, T w0, T wr, T& left,
       left = w0; right = w0;
       return;
   }
+}
+
+// ----------------------------------------- characteristic reconstruction --
+// *FACESLOPES (src/reconstructionF.ChF): limited slopes SL (lo side) and SR
+// (hi side) of a cell from its one-sided differences dSL = W - W(i-1),
+// dSR = W(i+1) - W, taken in characteristic variables.
+template <typename T>
+__device__ __forceinline__ void face_slopes(int lim, T dSL, T dSR, T& SL, T& SR) {
+  const T z = T(0), h = T(0.5);
+  T s = z;
+  switch (lim) {
+    case MFM_LIM_MINMOD: {  // MINMODFACESLOPES :157-177
+      T sg = copysign(T(1), dSR);
+      s = h * (sg * tmax(z, tmin(fabs(dSR), sg * dSL)));
+      break;
+    }
+    case MFM_LIM_VANLEER: {  // VANLEERFACESLOPES :843-881 (kappa = 1/3)
+      T pl = z, pr = z;
+      if (dSL * dSR > z) {
+        const T BT = T((3.0 - 1.0 / 3.0) / (1.0 - 1.0 / 3.0));
+        const T SL1 = T(0.25 * (1.0 - 1.0 / 3.0)), SL2 = T(0.25 * (1.0 + 1.0 / 3.0));
+        T a = tmin(fabs(dSR), fabs(BT * dSL));
+        T b = tmin(fabs(dSL), fabs(BT * dSR));
+        pl = SL1 * a + SL2 * b;
+        pr = SL1 * b + SL2 * a;
+        if (dSR < z) { pl = -pl; pr = -pr; }
+      }
+      SL = pl; SR = pr;
+      return;
+    }
+    case MFM_LIM_MC:  // MCFACESLOPES :664-694
+      if (dSL * dSR > z) {
+        T AR = fabs(dSR), AL = fabs(dSL);
+        if (T(3) * AL <= AR) s = dSL;
+        else if (AL <= T(3) * AR) s = T(0.25) * (dSR + dSL);
+        else s = dSR;
+      }
+      break;
+    case MFM_LIM_SUPERBEE:  // SUPERBEEFACESLOPES :356-388
+      if (dSL * dSR > z) {
+        T AR = fabs(dSR), AL = fabs(dSL);
+        if (AL <= h * AR) s = dSL;
+        else if (AL <= AR) s = h * dSR;
+        else if (AL <= AR + AR) s = h * dSL;
+        else s = dSR;
+      }
+      break;
+    case MFM_LIM_VANALBADA:  // VANALBADAFACESLOPES :591-620
+      if (dSL * dSR > z) {
+        const T dS = T(1.0e-6), dSS = T(1.0e-6) + T(1.0e-6);
+        s = h * (dSR * dSL + dS) * (dSR + dSL) / (dSR * dSR + dSL * dSL + dSS);
+      }
+      break;
+    case MFM_LIM_HARMONIC:  // HARMONICFACESLOPES :526-548
+      if (dSL * dSR > z) s = dSR * dSL / (dSL + dSR);
+      break;
+    case MFM_LIM_HYPERBEE:  // HYPERBEEFACESLOPES :466-490
+      if (dSL * dSR > z) {
+        T A = T(1) + tsqrt(dSR / dSL);
+        s = dSR / (A * A);
+        s = s + s;
+      }
+      break;
+    case MFM_LIM_KOREN: {  // KORENFACESLOPES :920-948
+      T pl = z, pr = z;
+      if (dSL * dSR > z) {
+        T H = dSL * dSR / (T(2) * (dSL * dSL + dSR * dSR + T(1.0e-10)) - dSL * dSR);
+        pl = (dSL + h * dSR) * H;
+        pr = (dSR + h * dSL) * H;
+      }
+      SL = pl; SR = pr;
+      return;
+    }
+    case MFM_LIM_VENKAT: {  // VENKATAKRISHNANFACESLOPES :990-1020
+      T pl = z, pr = z;
+      if (dSL * dSR > z) {
+        const T dS = T(1.0e-10), dS2 = T(1.0e-10 * 1.0e-10);
+        T H = (dSR + dSL) / (T(6) * (dSR * dSR + dSL * dSL + dS2));
+        pl = (dSR + T(2) * dSL) * (dSL + dS) * H;
+        pr = (dSL + T(2) * dSR) * (dSR + dS) * H;
+      }
+      SL = pl; SR = pr;
+      return;
+    }
+    case MFM_LIM_WENO3: {  // WENO3FACESLOPES :1138-1167
+      T BR = dSR * dSR + T(1.0e-6), AR = T(1) / (BR * BR);
+      T BL = dSL * dSL + T(1.0e-6), AL = T(1) / (BL * BL);
+      T H = AR / (AR + T(2) * AL);
+      SL = h * (dSL + H * (dSR - dSL));
+      H = AL / (AL + T(2) * AR);
+      SR = h * (dSR - H * (dSR - dSL));
+      return;
+    }
+    case MFM_LIM_WENO3YC: {  // WENO3YCFACESLOPES :1214-1247
+      T BR = dSR * dSR + T(1.0e-6), BL = dSL * dSL + T(1.0e-6);
+      T D2P = dSR - dSL;
+      D2P = D2P * D2P;
+      T AL = T(1) + D2P / BL, AR = T(1) + D2P / BR;
+      T H = AR / (AR + T(2) * AL);
+      SL = h * (dSL + H * (dSR - dSL));
+      H = AL / (AL + T(2) * AR);
+      SR = h * (dSR - H * (dSR - dSL));
+      return;
+    }
+    case MFM_LIM_CD:  // CDFACESLOPES :1081-1094
+      SL = h * dSL; SR = h * dSR;
+      return;
+    default:  // no limiter: the slopes stay as they are
+      SL = dSL; SR = dSR;
+      return;
+  }
+  SL = s; SR = s;
+}
+
+// Eigen-structure of the ideal-MHD system at one cell along dir, formed from
+// the cell's primitive state as in CHARANALYSISF / CHARSYNTHESISF
+// (srcIdeal/PatchIdealMHDF.ChF:1167-1505).  The fast / slow weights take the
+// degenerate branch below 1e-12 in the analysis (aF, aS) and below 1e-8 in
+// the synthesis (aF2, aS2).
+template <typename T>
+struct CharBasis {
+  T Rho, cc, c, af, as, aF, aS, aF2, aS2, betaY, betaZ, BX;
+};
+
+template <typename T>
+__device__ __forceinline__ void char_alpha(T dfs, T cc, T aas, T aaf, T eps, T& aF, T& aS) {
+  if (dfs < eps) { aF = T(1); aS = T(0); }
+  else if ((cc - aas) <= T(0)) { aF = T(0); aS = T(1); }
+  else if ((aaf - cc) <= T(0)) { aF = T(1); aS = T(0); }
+  else { aF = tsqrt((cc - aas) / dfs); aS = tsqrt((aaf - cc) / dfs); }
+}
+
+template <typename T, int dir>
+__device__ __forceinline__ CharBasis<T> char_basis(const Consts& C, const T* w) {
+  constexpr int bn = 5 + dir, bt1 = 5 + (dir + 1) % 3, bt2 = 5 + (dir + 2) % 3;
+  CharBasis<T> B;
+  const T PGas = fmaxr((T)C.smallp, w[4]);
+  const T Rho = fmaxr((T)C.smallr, w[0]);
+  const T BX = w[bn], BY = w[bt1], BZ = w[bt2];
+  const T cc = T(C.gamma) * PGas / Rho;
+  const T c = tsqrt(cc);
+  const T BX2 = BX * BX, BY2 = BY * BY, BZ2 = BZ * BZ;
+  const T BYZ2 = BY2 + BZ2;
+  const T BxBH = T(k1_8PI) * (BX2 + BYZ2);
+  const T bb = T(k1_4PI) * BX2 / Rho;
+  const T b = tsqrt(bb);
+  T aas, as, aaf, af;
+  if (BYZ2 < T(C.smallB) * T(C.smallB)) {
+    B.betaY = T(kSQRT_2);
+    B.betaZ = T(kSQRT_2);
+    if (bb > cc) { aaf = bb; aas = cc; af = b; as = c; }
+    else { aaf = cc; aas = bb; af = c; as = b; }
+  } else {
+    const T hyz = tsqrt(BYZ2);
+    B.betaY = BY / hyz;
+    B.betaZ = BZ / hyz;
+    const T szb = cc + (BxBH + BxBH) / Rho;
+    const T h = tsqrt(nonneg(szb * szb - T(4) * cc * bb));
+    aas = T(0.5) * (szb - h);
+    as = tsqrt(nonneg(aas));
+    aaf = T(0.5) * (szb + h);
+    af = tsqrt(aaf);
+  }
+  const T dfs = aaf - aas;
+  char_alpha<T>(dfs, cc, aas, aaf, T(1.0e-12), B.aF, B.aS);
+  char_alpha<T>(dfs, cc, aas, aaf, T(1.0e-8), B.aF2, B.aS2);
+  B.Rho = Rho; B.cc = cc; B.c = c; B.af = af; B.as = as; B.BX = BX;
+  return B;
+}
+
+// CHARANALYSISF (:1252-1325): primitive differences d (plasma components in
+// Cartesian order) -> characteristic variables a (entropy, div(B), two
+// Alfven, two fast, two slow).
+template <typename T, int dir>
+__device__ __forceinline__ void char_analysis(const CharBasis<T>& B, const T* d, T* a) {
+  constexpr int n = 1 + dir, t1 = 1 + (dir + 1) % 3, t2 = 1 + (dir + 2) % 3;
+  constexpr int bn = 5 + dir, bt1 = 5 + (dir + 1) % 3, bt2 = 5 + (dir + 2) % 3;
+  // SIGN(one, BX) with -0.0 counted as positive, as the synthesis's BX < 0
+  // test does, so that both transforms agree where the field is exactly zero
+  const T two = T(2), siBX = B.BX < T(0) ? T(-1) : T(1);
+  const T dP = d[4], dRho = d[0], dUn = d[n], dUt1 = d[t1], dUt2 = d[t2];
+  const T dBn = d[bn], dBt1 = d[bt1], dBt2 = d[bt2];
+  a[0] = dRho - dP / B.cc;
+  a[1] = dBn;
+  T h1 = T(kSQRT_2) * (B.betaY * dUt2 - B.betaZ * dUt1);
+  T h2 = siBX * (B.betaZ * dBt1 - B.betaY * dBt2) / (two * tsqrt(T(k2PI) * B.Rho));
+  a[2] = h1 + h2;
+  a[3] = h1 - h2;
+  h1 = dP / (two * B.Rho * B.cc);
+  h2 = dUn / (two * B.cc);
+  const T h3 = (B.betaY * dBt1 + B.betaZ * dBt2) / (T(4) * B.c * tsqrt(T(kPI) * B.Rho));
+  const T h4 = siBX * (B.betaY * dUt1 + B.betaZ * dUt2) / (two * B.cc);
+  const T aF = B.aF, aS = B.aS, af = B.af, as = B.as;
+  a[4] = aF * (h1 + af * h2) + aS * (h3 - as * h4);
+  a[5] = aF * (h1 - af * h2) + aS * (h3 + as * h4);
+  a[6] = aS * (h1 + as * h2) - aF * (h3 - af * h4);
+  a[7] = aS * (h1 - as * h2) - aF * (h3 + af * h4);
+}
+
+// CHARSYNTHESISF (:1438-1502): characteristic variables -> primitive
+// differences.
+template <typename T, int dir>
+__device__ __forceinline__ void char_synthesis(const CharBasis<T>& B, const T* a, T* d) {
+  constexpr int n = 1 + dir, t1 = 1 + (dir + 1) % 3, t2 = 1 + (dir + 2) % 3;
+  constexpr int bn = 5 + dir, bt1 = 5 + (dir + 1) % 3, bt2 = 5 + (dir + 2) % 3;
+  const T betaYX = B.BX < T(0) ? -B.betaY : B.betaY;
+  const T betaZX = B.BX < T(0) ? -B.betaZ : B.betaZ;
+  const T alpha3 = T(kSQRT_2) * a[2], alpha4 = T(kSQRT_2) * a[3];
+  const T a3p4 = alpha3 + alpha4, a3m4 = alpha3 - alpha4;
+  const T a5p6 = a[4] + a[5], a5m6 = a[4] - a[5];
+  const T a7p8 = a[6] + a[7], a7m8 = a[6] - a[7];
+  const T DN = T(2) * tsqrt(T(kPI) * B.Rho);
+  const T aF = B.aF2, aS = B.aS2;
+  const T HRho = B.Rho * (aF * a5p6 + aS * a7p8);
+  const T HU = aS * B.as * a5m6 - aF * B.af * a7m8;
+  const T HB = B.c * (aS * a5p6 - aF * a7p8);
+  d[0] = a[0] + HRho;
+  d[n] = aF * B.af * a5m6 + aS * B.as * a7m8;
+  d[t1] = -B.betaZ * a3p4 - betaYX * HU;
+  d[t2] = B.betaY * a3p4 - betaZX * HU;
+  d[4] = B.cc * HRho;
+  d[bn] = a[1];
+  d[bt1] = DN * (betaZX * a3m4 + B.betaY * HB);
+  d[bt2] = DN * (-betaYX * a3m4 + B.betaZ * HB);
 }
 
 // ---------------------------------------------------- cons -> prim (FP64) --

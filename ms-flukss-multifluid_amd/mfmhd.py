@@ -25,7 +25,7 @@ TIME_FIRST_ORDER, TIME_HANCOCK, TIME_RK2 = 0, 1, 2
 COORD_CARTESIAN, COORD_AXISYMMETRIC = 0, 1
 PROBLEM_NONE, PROBLEM_RIEMANN, PROBLEM_HELIO = 0, 1, 2
 PREC_FP64, PREC_FP32_FLUX = 0, 1
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 
 class Box(C.Structure):
@@ -77,6 +77,7 @@ class Params(C.Structure):
         ("reg_safe_z_top", C.c_double), ("reg_safe_z_bot", C.c_double),
         ("r0_dt_ignore", C.c_double), ("ce_scale_vel", C.c_double),
         ("ce_sigma_lindsay", C.c_double), ("ce_single_literals", C.c_int),
+        ("char_reconstruction", C.c_int),
     ]
 
 

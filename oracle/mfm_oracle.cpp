@@ -32,7 +32,7 @@ const double d_PI_2 = 1.57079632679489661923132169163975144;
 const double eos_AU = 1.5e13;
 const double netFluid2Dens = 1.0e-6, netFluid2Pres = 1.0e-8;
 const double netFluid3Dens = 1.0e-7, netFluid3Pres = 1.0e-9;
-const double half = 0.5, one = 1.0, two = 2.0, zero = 0.0;
+const double half = 0.5, one = 1.0, two = 2.0, four = 4.0, zero = 0.0;
 
 // srcIdeal/LGintegrator.H:10-46
 enum { WRHO = 0, WVELX = 1, WVELY = 2, WVELZ = 3, WPRES = 4, WBX = 5, WBY = 6,
@@ -1590,10 +1590,268 @@ void fill_ghost(const Eq& E, Fab& W, int dir) {
   else if (E.P->problem == MFM_PROBLEM_HELIO) helio_fill_ghost(E, W, dir);
 }
 
+// ========================================= characteristic reconstruction ==
+// *FACESLOPES (src/reconstructionF.ChF): limited left / right slopes of a
+// cell from its one-sided differences dSL = W - W(i-1), dSR = W(i+1) - W
+// (in characteristic variables when char_reconstruction is on).
+void face_slopes(int lim, double dSL, double dSR, double* SL, double* SR) {
+  double S = zero, SLOPL = zero, SLOPR = zero;
+  switch (lim) {
+    case MFM_LIM_CD:  // CDFACESLOPES :1081-1094
+      *SL = half * dSL; *SR = half * dSR;
+      return;
+    case MFM_LIM_MINMOD:  // MINMODFACESLOPES :157-177
+      S = half * minmod(dSR, dSL);
+      break;
+    case MFM_LIM_SUPERBEE:  // SUPERBEEFACESLOPES :356-388
+      if (dSL * dSR > zero) {
+        double AR = std::fabs(dSR), AL = std::fabs(dSL);
+        if (AL <= half * AR) S = dSL;
+        else if (AL <= AR) S = half * dSR;
+        else if (AL <= AR + AR) S = half * dSL;
+        else S = dSR;
+      }
+      break;
+    case MFM_LIM_HYPERBEE:  // HYPERBEEFACESLOPES :466-490
+      if (dSL * dSR > zero) {
+        double A = one + std::sqrt(dSR / dSL);
+        S = dSR / (A * A);
+        S = S + S;
+      }
+      break;
+    case MFM_LIM_HARMONIC:  // HARMONICFACESLOPES :526-548
+      if (dSL * dSR > zero) S = dSR * dSL / (dSL + dSR);
+      break;
+    case MFM_LIM_VANALBADA: {  // VANALBADAFACESLOPES :591-620
+      const double dSave = 1.0e-6, dSaveS = dSave + dSave;
+      if (dSL * dSR > zero)
+        S = half * (dSR * dSL + dSave) * (dSR + dSL) / (dSR * dSR + dSL * dSL + dSaveS);
+      break;
+    }
+    case MFM_LIM_MC:  // MCFACESLOPES :664-694
+      if (dSL * dSR > zero) {
+        double AR = std::fabs(dSR), AL = std::fabs(dSL);
+        if (3.0 * AL <= AR) S = dSL;
+        else if (AL <= 3.0 * AR) S = 0.25 * (dSR + dSL);
+        else S = dSR;
+      }
+      break;
+    case MFM_LIM_VANLEER: {  // VANLEERFACESLOPES :843-881, kappa = 1/3
+      const double dSLim = one / 3.0;
+      const double BT = (3.0 - dSLim) / (one - dSLim);
+      const double SL1 = 0.25 * (one - dSLim), SL2 = 0.25 * (one + dSLim);
+      if (dSL * dSR > zero) {
+        double SLm = std::min(std::fabs(dSR), std::fabs(BT * dSL));
+        double SRm = std::min(std::fabs(dSL), std::fabs(BT * dSR));
+        SLOPL = SL1 * SLm + SL2 * SRm;
+        SLOPR = SL1 * SRm + SL2 * SLm;
+        if (dSR < zero) { SLOPL = -SLOPL; SLOPR = -SLOPR; }
+      }
+      *SL = SLOPL; *SR = SLOPR;
+      return;
+    }
+    case MFM_LIM_KOREN: {  // KORENFACESLOPES :920-948
+      const double dSave = 1.0e-10;
+      if (dSL * dSR > zero) {
+        double H = dSL * dSR / (two * (dSL * dSL + dSR * dSR + dSave) - dSL * dSR);
+        SLOPL = (dSL + half * dSR) * H;
+        SLOPR = (dSR + half * dSL) * H;
+      }
+      *SL = SLOPL; *SR = SLOPR;
+      return;
+    }
+    case MFM_LIM_VENKAT: {  // VENKATAKRISHNANFACESLOPES :990-1020
+      const double dSave = 1.0e-10, dSave2 = dSave * dSave;
+      if (dSL * dSR > zero) {
+        double H = (dSR + dSL) / (6.0 * (dSR * dSR + dSL * dSL + dSave2));
+        SLOPL = (dSR + two * dSL) * (dSL + dSave) * H;
+        SLOPR = (dSL + two * dSR) * (dSR + dSave) * H;
+      }
+      *SL = SLOPL; *SR = SLOPR;
+      return;
+    }
+    case MFM_LIM_WENO3: {  // WENO3FACESLOPES :1138-1167
+      const double dSave = 1.0e-6;
+      double BR = dSR * dSR + dSave, AR = one / (BR * BR);
+      double BL = dSL * dSL + dSave, AL = one / (BL * BL);
+      double H = AR / (AR + two * AL);
+      *SL = half * (dSL + H * (dSR - dSL));
+      H = AL / (AL + two * AR);
+      *SR = half * (dSR - H * (dSR - dSL));
+      return;
+    }
+    case MFM_LIM_WENO3YC: {  // WENO3YCFACESLOPES :1214-1247
+      const double dSave = 1.0e-6;
+      double BR = dSR * dSR + dSave, BL = dSL * dSL + dSave;
+      double D2P = dSR - dSL;
+      D2P = D2P * D2P;
+      double AL = one + D2P / BL, AR = one + D2P / BR;
+      double H = AR / (AR + two * AL);
+      *SL = half * (dSL + H * (dSR - dSL));
+      H = AL / (AL + two * AR);
+      *SR = half * (dSR - H * (dSR - dSL));
+      return;
+    }
+    default:  // slopes::faceSlopes leaves unlimited slopes alone
+      *SL = dSL; *SR = dSR;
+      return;
+  }
+  *SL = S; *SR = S;
+}
+
+// Eigen-structure of the ideal-MHD system at one cell in direction dir, as
+// CHARANALYSISF / CHARSYNTHESISF (srcIdeal/PatchIdealMHDF.ChF:1167-1505)
+// form it from the cell's primitive state.  The fast/slow weights switch to
+// the degenerate branch below dfs_eps: 1e-12 in the analysis, 1e-8 in the
+// synthesis.
+struct CharBasis {
+  int n, t1, t2, bn, bt1, bt2;
+  double Rho, cc, c, af, as, alphaF, alphaS, betaY, betaZ, BX;
+};
+
+CharBasis char_basis(const Eq& E, const double* w, int dir, double dfs_eps) {
+  CharBasis b;
+  b.n = 1 + dir; b.t1 = 1 + (dir + 1) % 3; b.t2 = 1 + (dir + 2) % 3;
+  b.bn = 5 + dir; b.bt1 = 5 + (dir + 1) % 3; b.bt2 = 5 + (dir + 2) % 3;
+  double PGas = std::max(E.smallp, w[4]);
+  double Rho = std::max(E.smallr, w[0]);
+  double BX = w[b.bn], BY = w[b.bt1], BZ = w[b.bt2];
+  double cc = E.gamma * PGas / Rho;
+  double c = std::sqrt(cc);
+  double BX2 = BX * BX, BY2 = BY * BY, BZ2 = BZ * BZ;
+  double BYZ2 = BY2 + BZ2;
+  double BxBH = d_1_8PI * (BX2 + BYZ2);
+  double bb = d_1_4PI * BX2 / Rho;
+  double bs = std::sqrt(bb);
+  double aas, as, aaf, af, betaY, betaZ;
+  if (BYZ2 < E.smallB * E.smallB) {
+    betaY = d_SQRT_2;
+    betaZ = d_SQRT_2;
+    if (bb > cc) { aaf = bb; aas = cc; af = bs; as = c; }
+    else { aaf = cc; aas = bb; af = c; as = bs; }
+  } else {
+    double hyz = std::sqrt(BYZ2);
+    betaY = BY / hyz;
+    betaZ = BZ / hyz;
+    double szb = cc + (BxBH + BxBH) / Rho;
+    double h = std::sqrt(szb * szb - four * cc * bb);
+    aas = half * (szb - h);
+    as = std::sqrt(aas);
+    aaf = half * (szb + h);
+    af = std::sqrt(aaf);
+  }
+  double dfs = aaf - aas, alphaF, alphaS;
+  if (dfs < dfs_eps) { alphaF = one; alphaS = zero; }
+  else if ((cc - aas) <= zero) { alphaF = zero; alphaS = one; }
+  else if ((aaf - cc) <= zero) { alphaF = one; alphaS = zero; }
+  else {
+    alphaF = std::sqrt((cc - aas) / dfs);
+    alphaS = std::sqrt((aaf - cc) / dfs);
+  }
+  b.Rho = Rho; b.cc = cc; b.c = c; b.af = af; b.as = as;
+  b.alphaF = alphaF; b.alphaS = alphaS; b.betaY = betaY; b.betaZ = betaZ; b.BX = BX;
+  return b;
+}
+
+// CHARANALYSISF (:1252-1325): primitive differences d (components 0..7,
+// Cartesian order) -> characteristic variables a (entropy, div(B), two
+// Alfven, two fast, two slow).
+void char_analysis(const CharBasis& b, const double* d, double* a) {
+  // SIGN(one, BX) taken with a negative zero counting as positive, the rule
+  // of the synthesis's BX < 0 test: with the IEEE sign of -0.0 the two
+  // transforms stop being inverse where the normal field is exactly zero,
+  // and the result would hang on the sign of that zero, which the order of
+  // the update's floating-point operations decides.
+  double siBX = b.BX < zero ? -one : one;
+  double dP = d[4], dRho = d[0], dUn = d[b.n], dUt1 = d[b.t1], dUt2 = d[b.t2];
+  double dBn = d[b.bn], dBt1 = d[b.bt1], dBt2 = d[b.bt2];
+  a[0] = dRho - dP / b.cc;
+  a[1] = dBn;
+  double h1 = d_SQRT_2 * (b.betaY * dUt2 - b.betaZ * dUt1);
+  double h2 = siBX * (b.betaZ * dBt1 - b.betaY * dBt2) / (two * std::sqrt(two * d_PI * b.Rho));
+  a[2] = h1 + h2;
+  a[3] = h1 - h2;
+  h1 = dP / (two * b.Rho * b.cc);
+  h2 = dUn / (two * b.cc);
+  double h3 = (b.betaY * dBt1 + b.betaZ * dBt2) / (four * b.c * std::sqrt(d_PI * b.Rho));
+  double h4 = siBX * (b.betaY * dUt1 + b.betaZ * dUt2) / (two * b.cc);
+  const double aF = b.alphaF, aS = b.alphaS, af = b.af, as = b.as;
+  a[4] = aF * (h1 + af * h2) + aS * (h3 - as * h4);
+  a[5] = aF * (h1 - af * h2) + aS * (h3 + as * h4);
+  a[6] = aS * (h1 + as * h2) - aF * (h3 - af * h4);
+  a[7] = aS * (h1 - as * h2) - aF * (h3 + af * h4);
+}
+
+// CHARSYNTHESISF (:1438-1502): characteristic variables -> primitive
+// differences.
+void char_synthesis(const CharBasis& b, const double* a, double* d) {
+  double betaYX = b.betaY, betaZX = b.betaZ;
+  if (b.BX < zero) { betaYX = -b.betaY; betaZX = -b.betaZ; }
+  double alpha1 = a[0], alpha2 = a[1], alpha3 = a[2], alpha4 = a[3];
+  double alpha5 = a[4], alpha6 = a[5], alpha7 = a[6], alpha8 = a[7];
+  alpha3 = d_SQRT_2 * alpha3;
+  alpha4 = d_SQRT_2 * alpha4;
+  double alph3p4 = alpha3 + alpha4, alph3m4 = alpha3 - alpha4;
+  double alph5p6 = alpha5 + alpha6, alph5m6 = alpha5 - alpha6;
+  double alph7p8 = alpha7 + alpha8, alph7m8 = alpha7 - alpha8;
+  double DN = two * std::sqrt(d_PI * b.Rho);
+  const double aF = b.alphaF, aS = b.alphaS;
+  double HRho = b.Rho * (aF * alph5p6 + aS * alph7p8);
+  double HU = aS * b.as * alph5m6 - aF * b.af * alph7m8;
+  double HB = b.c * (aS * alph5p6 - aF * alph7p8);
+  d[0] = alpha1 + HRho;
+  d[b.n] = aF * b.af * alph5m6 + aS * b.as * alph7m8;
+  d[b.t1] = -b.betaZ * alph3p4 - betaYX * HU;
+  d[b.t2] = b.betaY * alph3p4 - betaZX * HU;
+  d[4] = b.cc * HRho;
+  d[b.bn] = alpha2;
+  d[b.bt1] = DN * (betaZX * alph3m4 + b.betaY * HB);
+  d[b.bt2] = DN * (-betaYX * alph3m4 + b.betaZ * HB);
+}
+
+// Reconstruction::faceValues, characteristic branch
+// (src/Reconstruction.cpp:178-204): INITIALSLOPES for every component,
+// EqSysMHDMF::charAnalysis on the plasma components 0..7
+// (srcIdeal/EqSysMHDMF.cpp:309-322), each component's faceSlopes, then
+// charSynthesis + FROMSLOPESTOVALUES (srcIdeal/EqSysMHDMF.cpp:325-346).
+// checkTVDCondition only acts where mhdam.check_tvd is set; it is not.
+void char_face_values(const Eq& E, const Fab& W, Fab& WM, Fab& WP, int dir,
+                      const Box& b) {
+  int io = dir == 0, jo = dir == 1, ko = dir == 2;
+  PFORBOX(b, i, j, k) {
+    double w[32], dL[32], dR[32], SL[32], SR[32];
+    for (int v = 0; v < E.nc; ++v) {  // INITIALSLOPES
+      double WPc = W(i, j, k, v);
+      w[v] = WPc;
+      dR[v] = W(i + io, j + jo, k + ko, v) - WPc;
+      dL[v] = WPc - W(i - io, j - jo, k - ko, v);
+    }
+    CharBasis ba = char_basis(E, w, dir, 1.0e-12);
+    double cL[8], cR[8];
+    char_analysis(ba, dL, cL);
+    char_analysis(ba, dR, cR);
+    for (int v = 0; v < 8; ++v) { dL[v] = cL[v]; dR[v] = cR[v]; }
+    for (int v = 0; v < E.nc; ++v) face_slopes(E.limiter(v), dL[v], dR[v], &SL[v], &SR[v]);
+    CharBasis bs = char_basis(E, w, dir, 1.0e-8);
+    double pL[8], pR[8];
+    char_synthesis(bs, SL, pL);
+    char_synthesis(bs, SR, pR);
+    for (int v = 0; v < 8; ++v) { SL[v] = pL[v]; SR[v] = pR[v]; }
+    for (int v = 0; v < E.nc; ++v) {  // FROMSLOPESTOVALUES
+      WM(i, j, k, v) = w[v] - SL[v];
+      WP(i, j, k, v) = w[v] + SR[v];
+    }
+  }
+}
+
 // Reconstruction::faceValues (src/Reconstruction.cpp:131-206), per variable
 void face_values(const Eq& E, const Fab& W, Fab& WM, Fab& WP, int dir,
                  const Box& b) {
   int io = dir == 0, jo = dir == 1, ko = dir == 2;
+  if (E.P->char_reconstruction) {
+    char_face_values(E, W, WM, WP, dir, b);
+    return;
+  }
   for (int v = 0; v < E.nc; ++v) {
     int lim = E.limiter(v);
     PFORBOX(b, i, j, k) {
@@ -2126,6 +2384,10 @@ int check_args(const mfm_params* p) {
   // FLUXESHANCOCK has no psi flux: the GLM system is not run with Hancock
   if (p->time_approx == MFM_TIME_HANCOCK && p->dedner) return MFM_ERR_UNSUPPORTED;
   if (p->coord_sys != MFM_COORD_CARTESIAN) return MFM_ERR_UNSUPPORTED;
+  if (p->char_reconstruction)  // no first-order (unlimited) plasma slopes
+    for (int v = 0; v < 8; ++v)
+      if (p->limiter[v] < MFM_LIM_CD || p->limiter[v] > MFM_LIM_WENO3YC)
+        return MFM_ERR_UNSUPPORTED;
   return MFM_OK;
 }
 
@@ -2432,6 +2694,30 @@ void oracle_flux_vanleer(const mfm_params* p, const double* wl,
 }
 void oracle_limiter(int lim, double wm, double w0, double wp, double* out) {
   limit(lim, wm, w0, wp, &out[0], &out[1]);
+}
+// Face values of the primitive FAB W over box b in direction dir
+// (Reconstruction::faceValues with the limiters and the characteristic
+// switch of p).  W must hold b grown by one cell in dir.
+int oracle_face_values(const mfm_params* p, mfm_fab Wf, int dir, mfm_box b,
+                       mfm_fab WMf, mfm_fab WPf) {
+  int rc = check_args(p);
+  if (rc) return rc;
+  Eq E(p, 1.0, 1.0);
+  Fab W, WM, WP;
+  W.wrap(Wf); WM.wrap(WMf); WP.wrap(WPf);
+  face_values(E, W, WM, WP, dir, mkbox(b));
+  return MFM_OK;
+}
+void oracle_face_slopes(int lim, double dsl, double dsr, double* out) {
+  face_slopes(lim, dsl, dsr, &out[0], &out[1]);
+}
+// which = 0: analysis of d (8 primitive differences) with the basis of the
+// primitive state w; which = 1: synthesis.
+void oracle_char_transform(const mfm_params* p, const double* w, int dir, int which,
+                           const double* in, double* out) {
+  Eq E(p, 1.0, 1.0);
+  if (which == 0) char_analysis(char_basis(E, w, dir, 1.0e-12), in, out);
+  else char_synthesis(char_basis(E, w, dir, 1.0e-8), in, out);
 }
 void oracle_cons_to_prim(const mfm_params* p, const double* u, double* w) {
   Eq E(p, 1.0, 1.0);

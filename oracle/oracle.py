@@ -54,6 +54,10 @@ def load():
     lib.oracle_flux_dedner.argtypes = [P, C.c_double, dp, dp, C.c_int, C.c_int, dp]
     lib.oracle_flux_vanleer.argtypes = [P, dp, dp, C.c_int, dp]
     lib.oracle_limiter.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp]
+    lib.oracle_face_slopes.argtypes = [C.c_int, C.c_double, C.c_double, dp]
+    lib.oracle_face_values.argtypes = [P, mfmhd.Fab, C.c_int, mfmhd.Box, mfmhd.Fab,
+                                      mfmhd.Fab]
+    lib.oracle_char_transform.argtypes = [P, dp, C.c_int, C.c_int, dp, dp]
     lib.oracle_cons_to_prim.argtypes = [P, dp, dp]
     lib.oracle_physical_flux.argtypes = [P, dp, C.c_int, dp]
     lib.oracle_charge_exchange.argtypes = [P, dp, C.c_int, C.c_double, dp]
@@ -193,6 +197,44 @@ def limiter(lim, wm, w0, wp):
     out = np.zeros(2)
     load().oracle_limiter(lim, wm, w0, wp, _ptr(out))
     return out[0], out[1]
+
+
+def face_slopes(lim, dsl, dsr):
+    """(SLOPL, SLOPR) of the *FACESLOPES limiter lim."""
+    out = np.zeros(2)
+    load().oracle_face_slopes(lim, dsl, dsr, _ptr(out))
+    return out[0], out[1]
+
+
+def face_values(p, W, lo, idir, box_lo, box_hi):
+    """(W-, W+) of the primitive array W (components, z, y, x; first cell at
+    lo) over box [box_lo, box_hi] in direction idir; zero outside the box."""
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    WM = np.zeros_like(W)
+    WP = np.zeros_like(W)
+    rc = load().oracle_face_values(C.byref(p), fab_of(W, lo), idir,
+                                   mfmhd.Box.of(box_lo, box_hi), fab_of(WM, lo),
+                                   fab_of(WP, lo))
+    if rc:
+        raise RuntimeError("oracle_face_values rc=%d" % rc)
+    return WM, WP
+
+
+def char_analysis(p, w, idir, d):
+    """Primitive differences d[0:8] -> characteristic variables at the
+    primitive state w (CHARANALYSISF)."""
+    out = np.zeros(8)
+    load().oracle_char_transform(C.byref(p), _ptr(_vec(w)), idir, 0,
+                                 _ptr(_vec(d, 8)), _ptr(out))
+    return out
+
+
+def char_synthesis(p, w, idir, a):
+    """Characteristic variables a[0:8] -> primitive differences (CHARSYNTHESISF)."""
+    out = np.zeros(8)
+    load().oracle_char_transform(C.byref(p), _ptr(_vec(w)), idir, 1,
+                                 _ptr(_vec(a, 8)), _ptr(out))
+    return out
 
 
 def cons_to_prim(p, u):

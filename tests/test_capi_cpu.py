@@ -116,6 +116,15 @@ def test_ctx_validation_without_device():
     p = mfmhd.default_params()
     p.time_approx = 3
     assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
+    p = mfmhd.default_params()            # characteristic slopes: FP64 build only
+    p.char_reconstruction = 1
+    p.precision = mfmhd.PREC_FP32_FLUX
+    assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
+    p = mfmhd.default_params()            # ... and a limiter on every plasma variable
+    p.char_reconstruction = 1
+    p.precision = mfmhd.PREC_FP64
+    p.limiter[4] = mfmhd.LIM_NONE
+    assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
     if mfmhd.device_count() == 0:
         p = mfmhd.default_params()
         assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_DEVICE

@@ -1,0 +1,149 @@
+"""GPU parity of characteristic reconstruction (mhdam.char_reconstruction = 1,
+the setting of the reference's Orszag-Tang deck main/inputs/orszag.inputs)
+against the oracle: FP64 <= 1e-12 relative per component and step (the
+FP32-flux build rejects the option).  Covers the interior plasma flux
+kernel, the boundary-face kernel (Riemann and heliospheric boundaries), the
+Hancock predictor and corrector, GLM, and every limiter."""
+import numpy as np
+import pytest
+
+import mfmhd
+from helpers import S4PI, grid_shape, periodic_fill, smooth_state
+from test_gpu_parity import check64, helio_setup, periodic_params, run_pair
+
+pytestmark = pytest.mark.gpu
+
+ALL_LIMS = [mfmhd.LIM_CD, mfmhd.LIM_MINMOD, mfmhd.LIM_SUPERBEE, mfmhd.LIM_VANALBADA,
+            mfmhd.LIM_MC, mfmhd.LIM_HARMONIC, mfmhd.LIM_HYPERBEE, mfmhd.LIM_KOREN,
+            mfmhd.LIM_VENKAT, mfmhd.LIM_VANLEER, mfmhd.LIM_WENO3, mfmhd.LIM_WENO3YC]
+
+
+def char_periodic(nf=1, lim=mfmhd.LIM_VANLEER, time=mfmhd.TIME_RK2, dedner=False,
+                  prec=mfmhd.PREC_FP64):
+    p = periodic_params(nf=nf, flux=mfmhd.FLUX_ROE8, lim=lim, time=time, dedner=dedner,
+                        eight=not dedner, prec=prec)
+    p.char_reconstruction = 1
+    return p
+
+
+@pytest.mark.parametrize("lim", ALL_LIMS)
+def test_periodic_patch_every_limiter(oracle_mod, lim):
+    print("char periodic limiter", lim, flush=True)
+    p = char_periodic(lim=lim)
+    ng = mfmhd.num_ghost_cells(p)
+    vlo, vhi = (0, 0, 0), (9, 7, 6)
+    lo, hi, shape = grid_shape(8, vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=40 + lim)
+    periodic_fill(U, ng)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.01, 0.05, want_F=True, want_divB=True)
+    check64(g, o, lo, vlo, vhi, U_in=U)
+    for d in range(3):
+        assert np.max(np.abs(g["F"][d] - o["F"][d])) <= 1e-12 * np.max(np.abs(o["F"][d]))
+
+
+def test_mixed_limiters_four_fluids(oracle_mod):
+    """A different limiter per characteristic variable, with neutral fluids
+    (which keep their component-wise minmod)."""
+    p = char_periodic(nf=4)
+    for v, lim in enumerate([mfmhd.LIM_MC, mfmhd.LIM_VANLEER, mfmhd.LIM_MINMOD, mfmhd.LIM_KOREN,
+                             mfmhd.LIM_SUPERBEE, mfmhd.LIM_WENO3, mfmhd.LIM_VANALBADA,
+                             mfmhd.LIM_HARMONIC]):
+        p.limiter[v] = lim
+    ng = mfmhd.num_ghost_cells(p)
+    nc = mfmhd.num_components(p)
+    vlo, vhi = (0, 0, 0), (11, 9, 7)
+    lo, hi, shape = grid_shape(nc, vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=3)
+    periodic_fill(U, ng)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.01, 0.05)
+    check64(g, o, lo, vlo, vhi, U_in=U)
+
+
+def test_glm_char(oracle_mod):
+    p = char_periodic(dedner=True, lim=mfmhd.LIM_MC)
+    ng = mfmhd.num_ghost_cells(p)
+    vlo, vhi = (0, 0, 0), (9, 9, 9)
+    lo, hi, shape = grid_shape(9, vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=5)
+    periodic_fill(U, ng)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.01, 0.05, ch=2.0, cp=0.5)
+    check64(g, o, lo, vlo, vhi)
+
+
+@pytest.mark.parametrize("nf,lim", [(1, mfmhd.LIM_VANLEER), (2, mfmhd.LIM_MINMOD),
+                                    (1, mfmhd.LIM_WENO3YC)])
+def test_hancock_char(oracle_mod, nf, lim):
+    """The Orszag-Tang deck's scheme: Hancock with characteristic slopes."""
+    print("char hancock", nf, lim, flush=True)
+    p = char_periodic(nf=nf, lim=lim, time=mfmhd.TIME_HANCOCK)
+    ng = mfmhd.num_ghost_cells(p)
+    nc = mfmhd.num_components(p)
+    vlo, vhi = (0, 0, 0), (11, 9, 7)
+    lo, hi, shape = grid_shape(nc, vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=60 + nf)
+    periodic_fill(U, ng)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.01, 0.05, want_F=True, want_divB=True)
+    check64(g, o, lo, vlo, vhi, U_in=U)
+
+
+@pytest.mark.parametrize("time", [mfmhd.TIME_RK2, mfmhd.TIME_HANCOCK])
+def test_helio_char(oracle_mod, time):
+    """Heliospheric box (boundary fluxes on every side, charge exchange)."""
+    n = 16
+    p, dx = helio_setup(oracle_mod, n, 4)
+    p.time_approx = time
+    p.char_reconstruction = 1
+    for v in range(8):
+        p.limiter[v] = mfmhd.LIM_VANLEER
+    nc = mfmhd.num_components(p)
+    ng = mfmhd.num_ghost_cells(p)
+    vlo, vhi = (0, 0, 0), (n - 1, n - 1, n - 1)
+    lo, hi, shape = grid_shape(nc, vlo, vhi, ng)
+    U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
+    dtn, _ = oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)
+    t = 0.0
+    for step in range(2):
+        print("char helio", time, step, flush=True)
+        g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.2 * dtn, dx, time=t)
+        check64(g, o, lo, vlo, vhi)
+        U = o["U"]
+        t += 0.2 * dtn
+        dtn = o["dt"]
+
+
+@pytest.mark.parametrize("time", [mfmhd.TIME_RK2, mfmhd.TIME_HANCOCK])
+def test_brio_wu_char(oracle_mod, time):
+    L = [1.0, 0, 0, 0, 1.0, 0.75 * S4PI, 1.0 * S4PI, 0]
+    R = [0.125, 0, 0, 0, 0.1, 0.75 * S4PI, -1.0 * S4PI, 0]
+    p = mfmhd.riemann_params(2.0, L, R, 0.5)
+    p.time_approx = time
+    p.char_reconstruction = 1
+    n = 64
+    p.domain_lo[:] = [0, 0, 0]
+    p.domain_hi[:] = [n - 1, 0, 0]
+    p.periodic[:] = [0, 1, 1]
+    dx = 1.0 / n
+    ng = mfmhd.num_ghost_cells(p)
+    for vlo, vhi in (((0, 0, 0), (n - 1, 0, 0)), ((30, 0, 0), (n - 1, 0, 0))):
+        lo, hi, shape = grid_shape(8, vlo, vhi, ng)
+        U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
+        dtn, _ = oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)
+        g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.8 * dtn, dx)
+        check64(g, o, lo, vlo, vhi)
+
+
+def test_char_rejects_unlimited_slopes():
+    p = char_periodic()
+    p.limiter[2] = mfmhd.LIM_NONE
+    with pytest.raises(mfmhd.MfmError) as e:
+        mfmhd.Context(p)
+    assert e.value.code == mfmhd.ERR_UNSUPPORTED
+
+
+def test_char_rejects_fp32_build():
+    """Characteristic slopes run in the FP64 build only: with FP32 primitives
+    the pressure rounding reaches the field slopes through the eigenvectors."""
+    p = char_periodic(prec=mfmhd.PREC_FP32_FLUX)
+    with pytest.raises(mfmhd.MfmError) as e:
+        mfmhd.Context(p)
+    assert e.value.code == mfmhd.ERR_UNSUPPORTED
