@@ -132,6 +132,11 @@ typedef struct mfm_params {
   /* plasma slopes in characteristic variables (Reconstruction::faceValues,*/
   /* src/Reconstruction.cpp:178-204)                                     */
   int    char_reconstruction;
+  /* mhdam.check_tvd (main/MHDAM.cpp:942-966), read only with             */
+  /* char_reconstruction: after the synthesis the face values of the     */
+  /* flagged variables are clipped by CHECKTVD (src/reconstructionF.ChF: */
+  /* 1248-1278).  1 = all plasma variables, 2 = density and pressure.    */
+  int    check_tvd;
 } mfm_params;
 
 /* sizeof(mfm_params), for FFI bindings to check their struct layout. */

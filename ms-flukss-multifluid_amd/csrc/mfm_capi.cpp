@@ -122,6 +122,7 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
     return MFM_ERR_UNSUPPORTED;
   }
   if (p->coord_sys != MFM_COORD_CARTESIAN) { why = "only Cartesian coordinates"; return MFM_ERR_UNSUPPORTED; }
+  if (p->check_tvd < 0 || p->check_tvd > 2) { why = "check_tvd must be 0, 1 or 2"; return MFM_ERR_ARG; }
   if (p->char_reconstruction && p->precision != MFM_PREC_FP64) {
     why = "char_reconstruction runs in the FP64 build only";
     return MFM_ERR_UNSUPPORTED;
@@ -160,6 +161,8 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
   C.problem = p->problem;
   for (int c = 0; c < kMaxComp; ++c) C.lim[c] = (int8_t)(c < 8 ? p->limiter[c] : MFM_LIM_MINMOD);
   C.char_rec = p->char_reconstruction ? 1 : 0;
+  C.tvd_mask = !C.char_rec ? 0 : (p->check_tvd == 1 ? 0xFF
+                                  : (p->check_tvd == 2 ? ((1 << 0) | (1 << 4)) : 0));
   C.plasma_lim = C.lim[0];
   for (int c = 1; c < 8 + C.dedner; ++c)
     if (C.lim[c] != C.lim[0]) C.plasma_lim = -1;

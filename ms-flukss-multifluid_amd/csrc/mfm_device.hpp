@@ -48,6 +48,7 @@ struct Consts {
   int8_t lim[kMaxComp];
   int plasma_lim;     // limiter shared by every plasma variable (psi incl.), or -1
   int char_rec;       // plasma slopes limited in characteristic variables
+  int tvd_mask;       // bit v: CHECKTVD clips the face values of variable v
   double gamma, hgamma, invhgamma, smallr, smallp, del2, smallB;
   double ch, cp;
   int dom_lo[3], dom_hi[3], phys[3];  // phys[d]: non-periodic physical dir
@@ -306,6 +307,25 @@ __device__ __forceinline__ void face_slopes(int lim, T dSL, T dSR, T& SL, T& SR)
       return;
   }
   SL = s; SR = s;
+}
+
+// minmod3 (src/reconstructionF.ChF:16-26)
+template <typename T>
+__device__ __forceinline__ T minmod3(T A, T B, T C) {
+  const T sg = copysign(T(1), A);
+  return sg * tmax(T(0), tmin(tmin(fabs(A), sg * B), sg * C));
+}
+
+// CHECKTVD (src/reconstructionF.ChF:1248-1278) on one face value: the
+// slope W - W- (lo) or W+ - W (hi) is clipped to the cell's one-sided
+// differences.
+template <typename T>
+__device__ __forceinline__ T tvd_lo(T wm, T w0, T dWL, T dWR) {
+  return w0 - minmod3(dWL, dWR, w0 - wm);
+}
+template <typename T>
+__device__ __forceinline__ T tvd_hi(T wp, T w0, T dWL, T dWR) {
+  return w0 + minmod3(dWL, dWR, wp - w0);
 }
 
 // Eigen-structure of the ideal-MHD system at one cell along dir, formed from

@@ -77,7 +77,7 @@ class Params(C.Structure):
         ("reg_safe_z_top", C.c_double), ("reg_safe_z_bot", C.c_double),
         ("r0_dt_ignore", C.c_double), ("ce_scale_vel", C.c_double),
         ("ce_sigma_lindsay", C.c_double), ("ce_single_literals", C.c_int),
-        ("char_reconstruction", C.c_int),
+        ("char_reconstruction", C.c_int), ("check_tvd", C.c_int),
     ]
 
 

@@ -1809,12 +1809,29 @@ void char_synthesis(const CharBasis& b, const double* a, double* d) {
   d[b.bt2] = DN * (-betaYX * alph3m4 + b.betaZ * HB);
 }
 
+// minmod3 (src/reconstructionF.ChF:16-26)
+inline double minmod3(double A, double B, double C) {
+  double signA = fsign1(A);
+  return signA * std::max(zero, std::min(std::min(std::fabs(A), signA * B), signA * C));
+}
+
+// Variables whose face values Reconstruction::checkTVDCondition clips
+// (slopes::setTVDCheck from mhdam.check_tvd, main/MHDAM.cpp:942-966: 1
+// flags RHO..PRES and BX..BZ and falls through to 2, which flags RHO and
+// PRES).
+bool tvd_checked(const mfm_params* P, int v) {
+  if (P->check_tvd == 1) return v < 8;
+  if (P->check_tvd == 2) return v == WRHO || v == WPRES;
+  return false;
+}
+
 // Reconstruction::faceValues, characteristic branch
 // (src/Reconstruction.cpp:178-204): INITIALSLOPES for every component,
 // EqSysMHDMF::charAnalysis on the plasma components 0..7
 // (srcIdeal/EqSysMHDMF.cpp:309-322), each component's faceSlopes, then
-// charSynthesis + FROMSLOPESTOVALUES (srcIdeal/EqSysMHDMF.cpp:325-346).
-// checkTVDCondition only acts where mhdam.check_tvd is set; it is not.
+// charSynthesis + FROMSLOPESTOVALUES (srcIdeal/EqSysMHDMF.cpp:325-346),
+// then checkTVDCondition (src/Reconstruction.cpp:236-250) on the variables
+// mhdam.check_tvd flags.
 void char_face_values(const Eq& E, const Fab& W, Fab& WM, Fab& WP, int dir,
                       const Box& b) {
   int io = dir == 0, jo = dir == 1, ko = dir == 2;
@@ -1840,6 +1857,16 @@ void char_face_values(const Eq& E, const Fab& W, Fab& WM, Fab& WP, int dir,
     for (int v = 0; v < E.nc; ++v) {  // FROMSLOPESTOVALUES
       WM(i, j, k, v) = w[v] - SL[v];
       WP(i, j, k, v) = w[v] + SR[v];
+    }
+    for (int v = 0; v < E.nc; ++v) {  // checkTVDCondition -> CHECKTVD :1248-1278
+      if (!tvd_checked(E.P, v)) continue;
+      double WPc = w[v];
+      double dWR = W(i + io, j + jo, k + ko, v) - WPc;
+      double dWL = WPc - W(i - io, j - jo, k - ko, v);
+      double dSL = WPc - WM(i, j, k, v);
+      double dSR = WP(i, j, k, v) - WPc;
+      WM(i, j, k, v) = WPc - minmod3(dWL, dWR, dSL);
+      WP(i, j, k, v) = WPc + minmod3(dWL, dWR, dSR);
     }
   }
 }
@@ -2384,6 +2411,7 @@ int check_args(const mfm_params* p) {
   // FLUXESHANCOCK has no psi flux: the GLM system is not run with Hancock
   if (p->time_approx == MFM_TIME_HANCOCK && p->dedner) return MFM_ERR_UNSUPPORTED;
   if (p->coord_sys != MFM_COORD_CARTESIAN) return MFM_ERR_UNSUPPORTED;
+  if (p->check_tvd < 0 || p->check_tvd > 2) return MFM_ERR_ARG;
   if (p->char_reconstruction)  // no first-order (unlimited) plasma slopes
     for (int v = 0; v < 8; ++v)
       if (p->limiter[v] < MFM_LIM_CD || p->limiter[v] > MFM_LIM_WENO3YC)

@@ -17,6 +17,9 @@ What fixes it, independently of its code:
   - when the two one-sided differences of a cell are multiples a, b of one
     right eigenvector r_k of that cell, the face values are
     W -+ slope(a, b) r_k with slope the limiter of variable k;
+  - check_tvd (CHECKTVD): the clipped face values lie between the cell
+    value and its neighbour, within the smaller one-sided difference, on
+    exactly the variables the option names; linear data pass unchanged;
   - whole steps: a uniform state stays, a periodic box conserves, the
     Brio-Wu shock tube keeps its plateaus.
 """
@@ -205,6 +208,65 @@ def test_single_wave_projection(oracle_mod, k, lim):
     sl, sr = oracle_mod.face_slopes(lim, a, b)
     assert np.allclose(WM[:, 0, 0, 1], w - sl * r, atol=1e-13, rtol=1e-12)
     assert np.allclose(WP[:, 0, 0, 1], w + sr * r, atol=1e-13, rtol=1e-12)
+
+
+def _rough_line(seed, nx=12):
+    rng = np.random.default_rng(seed)
+    W = np.zeros((8, 1, 1, nx))
+    base = np.array(STATES[0])
+    for c in range(8):
+        W[c, 0, 0, :] = base[c] + 0.2 * abs(base[c] + 0.5) * rng.standard_normal(nx)
+    W[0] = np.abs(W[0]) + 0.2
+    W[4] = np.abs(W[4]) + 0.2
+    return W
+
+
+def _tvd_violations(W, WM, WP, comps):
+    n = 0
+    for c in comps:
+        w = W[c, 0, 0]
+        for i in range(1, w.size - 1):
+            dl, dr = w[i] - w[i - 1], w[i + 1] - w[i]
+            for s in (w[i] - WM[c, 0, 0, i], WP[c, 0, 0, i] - w[i]):
+                if s * dl < -1e-15 or s * dr < -1e-15 or abs(s) > min(abs(dl), abs(dr)) + 1e-15:
+                    n += 1
+    return n
+
+
+@pytest.mark.parametrize("lim", [mfmhd.LIM_VANLEER, mfmhd.LIM_SUPERBEE, mfmhd.LIM_CD])
+def test_check_tvd_bounds_face_values(oracle_mod, lim):
+    """check_tvd = 1: every plasma face value lies between the cell value and
+    its neighbour on that side, within the smaller one-sided difference;
+    without the check the characteristic values break that somewhere;
+    check_tvd = 2 clips density and pressure and leaves the rest alone."""
+    W = _rough_line(11)
+    box = ((1, 0, 0), (10, 0, 0))
+    p = char_params(lim)
+    WM0, WP0 = oracle_mod.face_values(p, W, (0, 0, 0), 0, *box)
+    assert _tvd_violations(W, WM0, WP0, range(8)) > 0
+    p.check_tvd = 1
+    WM1, WP1 = oracle_mod.face_values(p, W, (0, 0, 0), 0, *box)
+    assert _tvd_violations(W, WM1, WP1, range(8)) == 0
+    p.check_tvd = 2
+    WM2, WP2 = oracle_mod.face_values(p, W, (0, 0, 0), 0, *box)
+    assert _tvd_violations(W, WM2, WP2, (0, 4)) == 0
+    for c in (1, 2, 3, 5, 6, 7):
+        assert np.array_equal(WM2[c], WM0[c]) and np.array_equal(WP2[c], WP0[c])
+    for c in (0, 4):
+        assert np.array_equal(WM2[c], WM1[c]) and np.array_equal(WP2[c], WP1[c])
+
+
+def test_check_tvd_leaves_linear_data(oracle_mod):
+    p = char_params(mfmhd.LIM_MC)
+    p.check_tvd = 1
+    base = np.array(STATES[0])
+    grad = np.array([0.01, 0.003, -0.002, 0.004, 0.02, 0.0, 0.005, -0.003])
+    W = _line(7, base, grad)
+    WM, WP = oracle_mod.face_values(p, W, (-1, 0, 0), 0, (0, 0, 0), (4, 0, 0))
+    for c in range(8):
+        v = W[c, 0, 0, 1:6]
+        assert np.allclose(WM[c, 0, 0, 1:6], v - 0.5 * grad[c], atol=1e-13, rtol=0)
+        assert np.allclose(WP[c, 0, 0, 1:6], v + 0.5 * grad[c], atol=1e-13, rtol=0)
 
 
 def test_uniform_state_unchanged(oracle_mod):

@@ -132,6 +132,45 @@ def test_brio_wu_char(oracle_mod, time):
         check64(g, o, lo, vlo, vhi)
 
 
+@pytest.mark.parametrize("tvd,time,lim", [(1, mfmhd.TIME_RK2, mfmhd.LIM_VANLEER),
+                                          (2, mfmhd.TIME_RK2, mfmhd.LIM_SUPERBEE),
+                                          (1, mfmhd.TIME_HANCOCK, mfmhd.LIM_MC)])
+def test_check_tvd(oracle_mod, tvd, time, lim):
+    """mhdam.check_tvd on top of the characteristic slopes (the reference's
+    rotor and blast decks), on rough data so that the clipping acts."""
+    print("char check_tvd", tvd, time, lim, flush=True)
+    p = char_periodic(lim=lim, time=time)
+    p.check_tvd = tvd
+    ng = mfmhd.num_ghost_cells(p)
+    vlo, vhi = (0, 0, 0), (9, 8, 7)
+    lo, hi, shape = grid_shape(8, vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=80 + tvd)
+    rng = np.random.default_rng(tvd)
+    U[1:4] *= 1 + 0.3 * rng.standard_normal(U[1:4].shape)
+    U[5:8] *= 1 + 0.3 * rng.standard_normal(U[5:8].shape)
+    periodic_fill(U, ng)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.002, 0.05, want_F=True)
+    check64(g, o, lo, vlo, vhi, U_in=U)
+    p.check_tvd = 0
+    o0 = oracle_mod.patch_update(p, U, lo, vlo, vhi, 0.002, 0.05)
+    assert not np.array_equal(o0["U"], o["U"])  # the clipping acted
+
+
+def test_helio_check_tvd(oracle_mod):
+    n = 16
+    p, dx = helio_setup(oracle_mod, n, 4)
+    p.char_reconstruction = 1
+    p.check_tvd = 1
+    nc = mfmhd.num_components(p)
+    ng = mfmhd.num_ghost_cells(p)
+    vlo, vhi = (0, 0, 0), (n - 1, n - 1, n - 1)
+    lo, hi, shape = grid_shape(nc, vlo, vhi, ng)
+    U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
+    dtn, _ = oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.2 * dtn, dx)
+    check64(g, o, lo, vlo, vhi)
+
+
 def test_char_rejects_unlimited_slopes():
     p = char_periodic()
     p.limiter[2] = mfmhd.LIM_NONE
