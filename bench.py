@@ -417,6 +417,82 @@ def hancock_bench(prec, n=256, steps=10, warmup=3, walk=None):
             "kernels": kern}
 
 
+def orszag_bench(nxy=512, nz=32, steps=10, warmup=3):
+    """The scheme of the reference's Orszag-Tang deck (main/inputs/
+    orszag.inputs: Hancock, characteristic slopes with check_tvd = 1, van
+    Leer on every variable, Roe 8-wave, cfl 0.5, max_dt_growth 1.1) on one
+    periodic nxy x nxy x nz box in FP64 -- the deck's finest level (64 x 64
+    x 4 refined three times by 2) as one patch, with the ghost exchange in
+    every step -- and the same box with component-wise slopes."""
+    out = {"box": [nxy, nxy, nz], "precision": "fp64"}
+    for label, char in (("characteristic", 1), ("componentwise", 0)):
+        p = mfmhd.orszag_params(1.66666666667)
+        p.spacedim = 3
+        p.nfluids = 1
+        p.time_approx = mfmhd.TIME_HANCOCK
+        p.char_reconstruction = char
+        p.check_tvd = 1
+        p.flux = mfmhd.FLUX_ROE8
+        p.eight_wave = 1
+        p.divb_method = 1
+        p.precision = mfmhd.PREC_FP64
+        for v in range(8):
+            p.limiter[v] = mfmhd.LIM_VANLEER
+        hi_v = (nxy - 1, nxy - 1, nz - 1)
+        for d in range(3):
+            p.domain_lo[d] = 0
+            p.domain_hi[d] = hi_v[d]
+            p.periodic[d] = 1
+        dx = 1.0 / nxy
+        ctx = mfmhd.Context(p)
+        ng = ctx.nghost
+        lo = (-ng,) * 3
+        hi = tuple(h + ng for h in hi_v)
+        U = mfmhd.DeviceArray((8, nz + 2 * ng, nxy + 2 * ng, nxy + 2 * ng))
+        Uf = mfmhd.fab(U.ptr, lo, hi, 8)
+        ctx.init_problem(Uf, mfmhd.Box.of(lo, hi), dx)
+        plan = mfmhd.FillPlan(p, [((0, 0, 0), hi_v)], interp_radius=ng)
+        valid = mfmhd.Box.of((0, 0, 0), hi_v)
+        ctx.exchange(plan, [Uf])
+        dtn, _ = ctx.compute_dt(Uf, valid, dx)
+        t, dt = 0.0, None
+
+        def step():
+            nonlocal t, dt, dtn
+            dt = 0.5 * dtn if dt is None else min(0.5 * dtn, 1.1 * dt)
+            ctx.exchange(plan, [Uf])
+            dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
+            t += dt
+
+        for _ in range(warmup):
+            step()
+        mfmhd.synchronize()
+        e0, e1 = mfmhd.Event(), mfmhd.Event()
+        e0.record()
+        for _ in range(steps):
+            step()
+        e1.record()
+        mfmhd.synchronize()
+        ms = e0.elapsed_ms(e1) / steps
+        ctx.set_timing(True)   # per-kernel times from a separate pass
+        for _ in range(3):
+            step()
+        mfmhd.synchronize()
+        kern = {}
+        for st, stage in ((0, "predictor"), (1, "corrector")):
+            for kind, name in enumerate(mfmhd.KT_NAMES):
+                kms, kn, kit = ctx.timing(st, kind)
+                if kn:
+                    kern["%s/%s" % (name, stage)] = {"avg_ms": kms / kn, "launches": kn}
+        out[label] = {"ms_per_step": ms,
+                      "cell_updates_per_s": float(nxy * nxy * nz) / (ms * 1e-3),
+                      "kernels": kern}
+        plan.close()
+        ctx.close()
+        U.free()
+    return out
+
+
 def copy_ceiling(gib=4, reps=5):
     """Device-to-device copy bandwidth (read + write bytes per second): the
     HBM rate a streaming kernel can reach on this device."""
@@ -456,6 +532,8 @@ def main():
                     help="skip the batched AMR patch-table measurement")
     ap.add_argument("--no-hancock", action="store_true",
                     help="skip the Hancock-scheme measurement of the same workload")
+    ap.add_argument("--no-orszag", action="store_true",
+                    help="skip the Orszag-Tang deck scheme (characteristic slopes) timing")
     ap.add_argument("--no-fill", action="store_true",
                     help="skip the level ghost-fill measurement")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -582,6 +660,8 @@ def main():
         out["hancock"] = hancock_bench(prec, n, args.steps, args.warmup, walk)
     if not args.no_fill:
         out["ghost_fill"] = fill_bench()
+    if not args.no_orszag:
+        out["orszag_char"] = orszag_bench()
     out["roofline"]["copy_ceiling_GBps"] = copy_ceiling() / 1e9
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
