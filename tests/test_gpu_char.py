@@ -171,6 +171,57 @@ def test_helio_check_tvd(oracle_mod):
     check64(g, o, lo, vlo, vhi)
 
 
+def test_level_table_char(oracle_mod):
+    """mfm_level_update_dev with characteristic slopes over uneven patches ==
+    one call per patch, and a patch == the oracle."""
+    from helpers import GpuPatch, gpu_update
+    p = char_periodic(nf=2, lim=mfmhd.LIM_VANLEER, time=mfmhd.TIME_HANCOCK)
+    p.check_tvd = 1
+    nc = mfmhd.num_components(p)
+    ng = mfmhd.num_ghost_cells(p)
+    rng = np.random.default_rng(19)
+    states = []
+    for q in range(4):
+        e = int(rng.choice([4, 8, 12]))
+        lo0 = rng.integers(-20, 20, size=3)
+        vlo, vhi = tuple(int(v) for v in lo0), tuple(int(v) + e - 1 for v in lo0)
+        lo, hi, shape = grid_shape(nc, vlo, vhi, ng)
+        states.append((smooth_state(p, shape, lo, seed=90 + q), lo, vlo, vhi))
+    ctx = mfmhd.Context(p)
+    singles = [gpu_update(ctx, U, lo, vlo, vhi, 0.01, 0.05) for U, lo, vlo, vhi in states]
+    pats = [GpuPatch(U, lo) for U, lo, _, _ in states]
+    ctx.update_level([(g.fab, mfmhd.Box.of(s[2], s[3])) for g, s in zip(pats, states)],
+                     0.01, 0.05)
+    mfmhd.synchronize()
+    for q, (g, s) in enumerate(zip(pats, singles)):
+        assert np.array_equal(g.numpy(), s["U"]), q
+    U, lo, vlo, vhi = states[2]
+    o = oracle_mod.patch_update(p, U, lo, vlo, vhi, 0.01, 0.05)
+    check64(singles[2], o, lo, vlo, vhi)
+    ctx.close()
+
+
+def test_server_char_class():
+    """The serving loop keeps characteristic slopes as part of the equation
+    class: a request with component-wise slopes is refused."""
+    from test_serve import req_params
+    n = 12
+    p, dx = req_params(n)
+    p.char_reconstruction = 1
+    srv = mfmhd.Server(p, 1 << 30)
+    q = mfmhd.Params.from_buffer_copy(p)
+    q.char_reconstruction = 0
+    with pytest.raises(mfmhd.MfmError):
+        srv.submit(q, dx, 2, 0.5)
+    rid = srv.submit(p, dx, 2, 0.5)
+    for _ in range(20):
+        srv.step()
+        if srv.info(rid)["state"] >= mfmhd.REQ_DONE:
+            break
+    assert srv.info(rid)["state"] == mfmhd.REQ_DONE, srv.error(rid)
+    srv.close()
+
+
 def test_char_rejects_unlimited_slopes():
     p = char_periodic()
     p.limiter[2] = mfmhd.LIM_NONE
