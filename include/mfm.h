@@ -432,6 +432,46 @@ int mfm_event_destroy(void* ev);
 int mfm_event_record(void* ev, void* stream);
 int mfm_event_elapsed_ms(void* start, void* stop, float* ms);
 
+/* ------------------------------------------------ Chombo HDF5 files ---- */
+/* Reader of the Chombo HDF5 checkpoint / plot files the reference restarts */
+/* from (AMR_MHDAM::setupForRestart, srcIdeal/AMR_MHDAM.cpp:242, 584-747;   */
+/* AMRLevelIdealMHD::readCheckpointHeader / readCheckpointLevel,            */
+/* srcIdeal/AMRLevelIdealMHDIO.cpp:36-200), without the HDF5 library: the   */
+/* file-format subset Chombo's writer produces with the library defaults    */
+/* (contiguous or compact datasets, symbol-table or compact groups).         */
+/* Chunked / compressed datasets and dense groups: MFM_ERR_UNSUPPORTED.      */
+typedef struct mfm_chombo_file mfm_chombo_file;
+int mfm_chombo_open(const char* path, mfm_chombo_file** out);
+int mfm_chombo_close(mfm_chombo_file* f);
+const char* mfm_chombo_last_error(const mfm_chombo_file* f);
+/* root header: num_levels, num_components, Chombo_global/SpaceDim, time    */
+int mfm_chombo_header(mfm_chombo_file* f, int* num_levels, int* num_components,
+                      int* space_dim, double* time);
+/* "component_<c>" (the state names readCheckpointHeader checks)           */
+const char* mfm_chombo_component_name(const mfm_chombo_file* f, int comp);
+/* level_<l>: box count, dx, dt, ref_ratio, prob_domain, outputGhost       */
+int mfm_chombo_level(mfm_chombo_file* f, int level, int* nboxes, double* dx, double* dt,
+                     int* ref_ratio, mfm_box* prob_domain, int ghost[3]);
+int mfm_chombo_boxes(mfm_chombo_file* f, int level, mfm_box* boxes, int capacity);
+/* values of box b: (box grown by ghost) cells x num_components doubles,   */
+/* component-major, x fastest -- an mfm_fab's layout                       */
+int mfm_chombo_read_box(mfm_chombo_file* f, int level, int box, double* dst, size_t count);
+
+/* The HDF5 layer underneath, for inspection (paths like "/level_0/boxes") */
+typedef struct mfm_h5_file mfm_h5_file;
+int mfm_h5_open(const char* path, mfm_h5_file** out);
+int mfm_h5_close(mfm_h5_file* f);
+const char* mfm_h5_last_error(const mfm_h5_file* f);
+/* type_class: 0 integer, 1 floating point, 3 string, 6 compound; dims[8]  */
+int mfm_h5_dataset_info(mfm_h5_file* f, const char* path, int* rank, long* dims,
+                        int* type_class, size_t* type_size);
+int mfm_h5_dataset_read(mfm_h5_file* f, const char* path, void* dst, size_t bytes);
+int mfm_h5_attribute_read(mfm_h5_file* f, const char* object_path, const char* name,
+                          void* dst, size_t capacity, int* type_class, size_t* bytes);
+int mfm_h5_group_size(mfm_h5_file* f, const char* path, int* count);
+int mfm_h5_group_name(mfm_h5_file* f, const char* path, int index, char* name,
+                      size_t capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,7 @@ UNITS = [
     ("mfm_fill.hip", ["-ffp-contract=off"]),
     ("mfm_sync.hip", ["-ffp-contract=off"]),
     ("mfm_fill_plan.cpp", ["-ffp-contract=off"]),
+    ("mfm_chombo.cpp", []),
 ]
 
 

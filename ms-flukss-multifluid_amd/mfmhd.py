@@ -109,6 +109,12 @@ EXPORTS = [
     "mfm_flux_register_set_zero", "mfm_flux_register_increment_coarse",
     "mfm_flux_register_increment_fine", "mfm_flux_register_reflux",
     "mfm_level_average_down_dev",
+    "mfm_chombo_open", "mfm_chombo_close", "mfm_chombo_last_error",
+    "mfm_chombo_header", "mfm_chombo_component_name", "mfm_chombo_level",
+    "mfm_chombo_boxes", "mfm_chombo_read_box",
+    "mfm_h5_open", "mfm_h5_close", "mfm_h5_last_error", "mfm_h5_dataset_info",
+    "mfm_h5_dataset_read", "mfm_h5_attribute_read", "mfm_h5_group_size",
+    "mfm_h5_group_name",
 ]
 
 REQ_PENDING, REQ_ACTIVE, REQ_DONE, REQ_FAILED = range(4)
@@ -211,6 +217,28 @@ def load(path=LIB_PATH):
     lib.mfm_flux_register_increment_fine.argtypes = [vp, vp, FP, dbl, vp]
     lib.mfm_flux_register_reflux.argtypes = [vp, vp, FP, dbl, dbl, vp]
     lib.mfm_level_average_down_dev.argtypes = [vp, vp, FP, FP, dbl, vp]
+    ip, sz = C.POINTER(it), C.c_size_t
+    lib.mfm_chombo_open.argtypes = [C.c_char_p, C.POINTER(vp)]
+    lib.mfm_chombo_close.argtypes = [vp]
+    lib.mfm_chombo_last_error.argtypes = [vp]
+    lib.mfm_chombo_last_error.restype = C.c_char_p
+    lib.mfm_chombo_header.argtypes = [vp, ip, ip, ip, C.POINTER(dbl)]
+    lib.mfm_chombo_component_name.argtypes = [vp, it]
+    lib.mfm_chombo_component_name.restype = C.c_char_p
+    lib.mfm_chombo_level.argtypes = [vp, it, ip, C.POINTER(dbl), C.POINTER(dbl), ip, BP, ip]
+    lib.mfm_chombo_boxes.argtypes = [vp, it, BP, it]
+    lib.mfm_chombo_read_box.argtypes = [vp, it, it, C.POINTER(dbl), sz]
+    lib.mfm_h5_open.argtypes = [C.c_char_p, C.POINTER(vp)]
+    lib.mfm_h5_close.argtypes = [vp]
+    lib.mfm_h5_last_error.argtypes = [vp]
+    lib.mfm_h5_last_error.restype = C.c_char_p
+    lib.mfm_h5_dataset_info.argtypes = [vp, C.c_char_p, ip, C.POINTER(C.c_long), ip,
+                                        C.POINTER(sz)]
+    lib.mfm_h5_dataset_read.argtypes = [vp, C.c_char_p, vp, sz]
+    lib.mfm_h5_attribute_read.argtypes = [vp, C.c_char_p, C.c_char_p, vp, sz, ip,
+                                          C.POINTER(sz)]
+    lib.mfm_h5_group_size.argtypes = [vp, C.c_char_p, ip]
+    lib.mfm_h5_group_name.argtypes = [vp, C.c_char_p, it, C.c_char_p, sz]
     lib.mfm_dev_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
     lib.mfm_dev_free.argtypes = [C.c_void_p]
     lib.mfm_dev_memcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
@@ -727,3 +755,124 @@ class Server:
                                               C.byref(cu), C.byref(busy)))
         return {"p50_ms": p50.value, "p99_ms": p99.value, "iterations": it.value,
                 "cell_updates": cu.value, "busy_ms": busy.value}
+
+
+class H5File:
+    """Read-only view of an HDF5 file through the library's format reader
+    (mfm_h5_*): dataset shapes and values, attributes, group members."""
+
+    CLASS_INT, CLASS_FLOAT, CLASS_STRING, CLASS_COMPOUND = 0, 1, 3, 6
+
+    def __init__(self, path):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.mfm_h5_open(os.fsencode(path), C.byref(h))
+        if rc != OK:
+            raise MfmError(rc, "cannot read %s as HDF5" % path)
+        self.h = h
+
+    def _check(self, rc):
+        if rc != OK:
+            raise MfmError(rc, self.lib.mfm_h5_last_error(self.h).decode())
+
+    def members(self, path="/"):
+        n = C.c_int()
+        self._check(self.lib.mfm_h5_group_size(self.h, path.encode(), C.byref(n)))
+        out = []
+        for i in range(n.value):
+            buf = C.create_string_buffer(4096)
+            self._check(self.lib.mfm_h5_group_name(self.h, path.encode(), i, buf, 4096))
+            out.append(buf.value.decode())
+        return out
+
+    def dataset_info(self, path):
+        rank, cls, size = C.c_int(), C.c_int(), C.c_size_t()
+        dims = (C.c_long * 8)()
+        self._check(self.lib.mfm_h5_dataset_info(self.h, path.encode(), C.byref(rank), dims,
+                                                 C.byref(cls), C.byref(size)))
+        return tuple(dims[:rank.value]), cls.value, size.value
+
+    def read(self, path, dtype):
+        import numpy as np
+        shape, cls, size = self.dataset_info(path)
+        a = np.empty(shape, dtype=dtype)
+        if a.itemsize != size:
+            raise MfmError(ERR_SHAPE, "element size %d, dataset has %d" % (a.itemsize, size))
+        self._check(self.lib.mfm_h5_dataset_read(self.h, path.encode(),
+                                                 a.ctypes.data_as(C.c_void_p), a.nbytes))
+        return a
+
+    def attribute(self, obj_path, name):
+        """(type class, raw bytes) of an attribute."""
+        cls, nb = C.c_int(), C.c_size_t()
+        self._check(self.lib.mfm_h5_attribute_read(self.h, obj_path.encode(), name.encode(),
+                                                   None, 0, C.byref(cls), C.byref(nb)))
+        buf = C.create_string_buffer(max(1, nb.value))
+        self._check(self.lib.mfm_h5_attribute_read(self.h, obj_path.encode(), name.encode(),
+                                                   buf, nb.value, C.byref(cls), C.byref(nb)))
+        return cls.value, buf.raw[:nb.value]
+
+    def close(self):
+        if self.h:
+            self.lib.mfm_h5_close(self.h)
+            self.h = None
+
+
+class ChomboFile:
+    """A Chombo HDF5 checkpoint or plot file (mfm_chombo_*): header, levels,
+    boxes and the FAB of each box in the device API's layout."""
+
+    def __init__(self, path):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.mfm_chombo_open(os.fsencode(path), C.byref(h))
+        if rc != OK:
+            raise MfmError(rc, "cannot read %s as a Chombo file" % path)
+        self.h = h
+        nl, nc, sd, t = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        self._check(self.lib.mfm_chombo_header(self.h, C.byref(nl), C.byref(nc), C.byref(sd),
+                                               C.byref(t)))
+        self.num_levels, self.num_components = nl.value, nc.value
+        self.space_dim, self.time = sd.value, t.value
+        self.component_names = [self.lib.mfm_chombo_component_name(self.h, c).decode()
+                                for c in range(self.num_components)]
+
+    def _check(self, rc):
+        if rc != OK:
+            raise MfmError(rc, self.lib.mfm_chombo_last_error(self.h).decode())
+
+    def level(self, lev):
+        nb, dx, dt, rr = C.c_int(), C.c_double(), C.c_double(), C.c_int()
+        dom = Box()
+        g = (C.c_int * 3)()
+        self._check(self.lib.mfm_chombo_level(self.h, lev, C.byref(nb), C.byref(dx), C.byref(dt),
+                                              C.byref(rr), C.byref(dom), g))
+        return {"nboxes": nb.value, "dx": dx.value, "dt": dt.value, "ref_ratio": rr.value,
+                "prob_domain": (tuple(dom.lo), tuple(dom.hi)), "ghost": tuple(g)}
+
+    def boxes(self, lev):
+        n = self.level(lev)["nboxes"]
+        arr = (Box * max(1, n))()
+        self._check(self.lib.mfm_chombo_boxes(self.h, lev, arr, n))
+        return [(tuple(arr[i].lo), tuple(arr[i].hi)) for i in range(n)]
+
+    def read_box(self, lev, b):
+        """(FAB values as (nc, nz, ny, nx), lo corner of the grown box)."""
+        import numpy as np
+        info = self.level(lev)
+        lo, hi = self.boxes(lev)[b]
+        g = info["ghost"]
+        glo = tuple(lo[d] - g[d] for d in range(3))
+        ghi = tuple(hi[d] + g[d] for d in range(3))
+        shape = (self.num_components, ghi[2] - glo[2] + 1, ghi[1] - glo[1] + 1,
+                 ghi[0] - glo[0] + 1)
+        a = np.empty(shape)
+        self._check(self.lib.mfm_chombo_read_box(self.h, lev, b,
+                                                 a.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 a.size))
+        return a, glo
+
+    def close(self):
+        if self.h:
+            self.lib.mfm_chombo_close(self.h)
+            self.h = None
