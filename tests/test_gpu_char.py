@@ -222,6 +222,37 @@ def test_server_char_class():
     srv.close()
 
 
+@pytest.mark.parametrize("dedner,time", [(True, mfmhd.TIME_RK2), (False, mfmhd.TIME_HANCOCK),
+                                         (False, mfmhd.TIME_FIRST_ORDER)])
+def test_orszag_tang_2d_char(oracle_mod, dedner, time):
+    """2-D Orszag-Tang with characteristic slopes and check_tvd: with GLM
+    cleaning (RK2), with the 8-wave Hancock scheme of the deck, and with
+    the first-order time scheme."""
+    print("char 2-D orszag", dedner, time, flush=True)
+    p = periodic_params(nf=1, flux=mfmhd.FLUX_ROE8, lim=mfmhd.LIM_VANLEER, dims=2,
+                        dedner=dedner, eight=not dedner, time=time)
+    p.char_reconstruction = 1
+    p.check_tvd = 1
+    n = 32
+    ng = mfmhd.num_ghost_cells(p)
+    nc = mfmhd.num_components(p)
+    vlo, vhi = (0, 0, 0), (n - 1, n - 1, 0)
+    lo, hi, shape = grid_shape(nc, vlo, vhi, ng, dims=2)
+    dx = 1.0 / n
+    U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
+    dtn, _ = oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)
+    ch = dx / dtn
+    cp = np.sqrt(ch * dx)
+    for step in range(2):
+        periodic_fill(U, ng, dims=2)
+        g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.5 * dtn, dx, ch=ch, cp=cp)
+        check64(g, o, lo, vlo, vhi)
+        U = o["U"]
+        dtn = o["dt"]
+        ch = dx / dtn
+        cp = np.sqrt(ch * dx)
+
+
 def test_char_rejects_unlimited_slopes():
     p = char_periodic()
     p.limiter[2] = mfmhd.LIM_NONE
