@@ -12,6 +12,7 @@ advanced in place; dt follows the CFL condition from the previous step.
 Prints one JSON line (metric cell-updates/s, roofline and CPU baseline).
 """
 import argparse
+import math
 import json
 import os
 import sys
@@ -30,18 +31,33 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # helio03.inputs): cfl 0.2, initial_cfl 0.1, max_dt_growth 1.01
 CFL, CFL0, DT_GROWTH = 0.2, 0.1, 1.01
 
+# Solar wind of the heliospheric workload: 7 /cc at the inner sphere r = 0.1
+# (AU).  The setter takes the density at 1 AU and scales it by r^-2.  (With
+# 7 /cc at 1 AU, i.e. 700 /cc at the sphere, the explicit charge-exchange
+# source of the third neutral population is unstable at this dt and the
+# state overflows within a few steps.)
+SUN_N_1AU = 0.07
+
+
+def finite_dt(dtn, where):
+    """The next dt is the minimum over all cells and propagates NaN: a
+    non-finite value means the state itself has gone non-finite."""
+    if not math.isfinite(dtn) or dtn <= 0.0:
+        raise RuntimeError("%s: state went non-finite (next dt %r)" % (where, dtn))
+    return dtn
+
 # HBM bytes per k_update launch (average over the two RK2 stages) measured
 # with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the default workload
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 from a 1 GiB copy; summary in
 # profiles/k_update_helio256_traffic.md).  Valid for that workload only.
-TRAFFIC = {(256, "fp32flux"): 14052722890.0}
+TRAFFIC = {(256, "fp32flux"): 15141000000.0}
 
 
 def c3_params(n, prec):
     dx = 1.0 / n
     p = mfmhd.helio_params(lismN=0.06, lismV=2.6e6, lismT=6527.0, lismB=3e-6,
                            lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0),
-                           lismUdir=(-1.0, 0.0, 0.0), sunN=7.0, sunV=4.5e7,
+                           lismUdir=(-1.0, 0.0, 0.0), sunN=SUN_N_1AU, sunV=4.5e7,
                            sunXYZ=(0.5, 0.5, 0.5), R0=0.1, initR=0.3,
                            netN=0.1, dx=dx, num_cells=(n, n, n))
     p.nfluids = 4
@@ -103,7 +119,7 @@ def serve_bench(prec, n_req=48, steps=20, rate_hz=400.0, seed=7):
         dx = 1.0 / n
         p = mfmhd.helio_params(lismN=0.06, lismV=2.6e6, lismT=6527.0, lismB=lismB,
                                lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0),
-                               lismUdir=(-1.0, 0.0, 0.0), sunN=7.0, sunV=sunV,
+                               lismUdir=(-1.0, 0.0, 0.0), sunN=SUN_N_1AU, sunV=sunV,
                                sunXYZ=(0.5, 0.5, 0.5), R0=0.15, initR=0.3,
                                netN=0.1, dx=dx, num_cells=(n, n, n))
         p.nfluids = 4
@@ -166,7 +182,7 @@ def amr_bench(prec, n_patches=3000, levels=3, steps=5, warmup=1, seed=11):
         dx = 1.0 / n_dom
         p = mfmhd.helio_params(lismN=0.06, lismV=2.6e6, lismT=6527.0, lismB=3e-6,
                                lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0),
-                               lismUdir=(-1.0, 0.0, 0.0), sunN=7.0, sunV=4.5e7,
+                               lismUdir=(-1.0, 0.0, 0.0), sunN=SUN_N_1AU, sunV=4.5e7,
                                sunXYZ=(0.5, 0.5, 0.5), R0=0.1, initR=0.3,
                                netN=0.1, dx=dx, num_cells=(n_dom,) * 3)
         p.nfluids = 4
@@ -244,7 +260,7 @@ def fill_bench(box=16, per_side=10, levels=3, reps=5):
         dx = 1.0 / n_dom
         p = mfmhd.helio_params(lismN=0.06, lismV=2.6e6, lismT=6527.0, lismB=3e-6,
                                lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0),
-                               lismUdir=(-1.0, 0.0, 0.0), sunN=7.0, sunV=4.5e7,
+                               lismUdir=(-1.0, 0.0, 0.0), sunN=SUN_N_1AU, sunV=4.5e7,
                                sunXYZ=(0.5, 0.5, 0.5), R0=0.1, initR=0.3,
                                netN=0.1, dx=dx, num_cells=(n_dom,) * 3)
         p.nfluids = 4
@@ -391,7 +407,7 @@ def hancock_bench(prec, n=256, steps=10, warmup=3, walk=None):
     t, dt = 0.0, None
     for _ in range(warmup):
         dt = CFL0 * dtn if dt is None else min(CFL * dtn, DT_GROWTH * dt)
-        dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
+        dtn = finite_dt(ctx.update_state(Uf, valid, dt, dx, t)[0], "update")
         t += dt
     mfmhd.synchronize()
     ctx.set_timing(True)
@@ -399,7 +415,7 @@ def hancock_bench(prec, n=256, steps=10, warmup=3, walk=None):
     e0.record()
     for _ in range(steps):
         dt = min(CFL * dtn, DT_GROWTH * dt)
-        dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
+        dtn = finite_dt(ctx.update_state(Uf, valid, dt, dx, t)[0], "update")
         t += dt
     e1.record()
     mfmhd.synchronize()
@@ -461,7 +477,7 @@ def orszag_bench(nxy=512, nz=32, steps=10, warmup=3):
             nonlocal t, dt, dtn
             dt = 0.5 * dtn if dt is None else min(0.5 * dtn, 1.1 * dt)
             ctx.exchange(plan, [Uf])
-            dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
+            dtn = finite_dt(ctx.update_state(Uf, valid, dt, dx, t)[0], "update")
             t += dt
 
         for _ in range(warmup):
@@ -571,7 +587,7 @@ def main():
 
     for _ in range(args.warmup):
         dt = next_dt(dtn, dt)
-        dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
+        dtn = finite_dt(ctx.update_state(Uf, valid, dt, dx, t)[0], "update")
         t += dt
     mfmhd.synchronize()
     # per-kernel HIP events on the launch stream over the timed region
@@ -581,7 +597,7 @@ def main():
     w0 = time.perf_counter()
     for _ in range(args.steps):
         dt = next_dt(dtn, dt)
-        dtn, _ = ctx.update_state(Uf, valid, dt, dx, t)
+        dtn = finite_dt(ctx.update_state(Uf, valid, dt, dx, t)[0], "update")
         t += dt
     e1.record()
     mfmhd.synchronize()
@@ -626,8 +642,9 @@ def main():
         "dtype": "f64 state, %s fluxes" % ("f32" if prec else "f64"),
         "data": "synthetic (HELIOINIT state generated on the device)",
         "config": {"workload": "3-D multi-fluid heliospheric box %d^3, plasma + 3 "
-                               "neutrals (23 comps), RK2, Roe8, minmod, 8-wave, "
-                               "charge exchange, single patch" % n,
+                               "neutrals (23 comps), solar wind from an inner sphere r=0.1 "
+                               "(7/cc, 450 km/s) into the LISM flow (26 km/s, 3 uG at 45 deg), "
+                               "RK2, Roe8, minmod, 8-wave, charge exchange, single patch" % n,
                    "cells": int(cells), "components": nc, "precision": args.precision,
                    "dt_control": "cfl %.2g, initial_cfl %.2g, max_dt_growth %.2f" % (args.cfl, CFL0, DT_GROWTH),
                    "walk": {"row_tile": walk[0], "xcd_order": walk[1]}},
