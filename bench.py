@@ -220,6 +220,7 @@ def amr_bench(prec, n_patches=3000, levels=3, steps=5, warmup=1, seed=11):
                 L["dtn"] = min(L["ctx"].compute_dt(f, b, L["dx"])[0] for f, b in L["table"])
             dt = CFL0 * L["dtn"] if L["dt"] is None else min(CFL * L["dtn"], DT_GROWTH * L["dt"])
             ld, _, _, _ = L["ctx"].update_level(L["table"], dt, L["dx"], L["t"])
+            ld = finite_dt(ld, "batched AMR level")
             L["dt"], L["dtn"] = dt, ld
             L["t"] += dt
 
