@@ -379,7 +379,8 @@ int mfm_fill_time_coef(double time, double t_coarse_new, double dt_coarse,
 /* ---- ensemble serving ----------------------------------------------------
  * No counterpart in the reference, which runs one simulation per process.
  * A server owns independent solves of one equation class (spacedim,
- * nfluids, dedner, time scheme, precision of `cls`).  Each request brings
+ * nfluids, dedner, Riemann solver family (Roe or Rusanov), time scheme,
+ * precision and characteristic slopes of `cls`).  Each request brings
  * its own parameters (problem constants and domain = its grid), dx, step
  * count and CFL number.  mfm_server_step admits pending requests in arrival
  * order while their device bytes (state + update scratch) fit the HBM

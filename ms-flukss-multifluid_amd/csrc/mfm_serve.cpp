@@ -1,6 +1,6 @@
 // mfm_serve.cpp -- ensemble serving: independent solves of one equation
-// class (dimension, fluids, GLM, time scheme, precision,
-// characteristic or component-wise slopes) admitted by an HBM
+// class (dimension, fluids, GLM, Riemann solver family, time scheme,
+// precision, characteristic or component-wise slopes) admitted by an HBM
 // budget and advanced together, one RK2 step per iteration, in the same
 // batched launches (continuous batching).  Each request keeps its state on
 // the device from admission until the caller fetches and releases it.
@@ -85,8 +85,12 @@ size_t request_bytes(const mfm_params& cls, const Consts& C, const HBox& fab, co
   return b + 4096;  // tables
 }
 
+// Riemann solver family (the plasma flux kernel is compiled per family)
+int solver(const mfm_params& p) { return p.dedner ? 2 : ((p.flux == 2 || p.flux == 4) ? 1 : 0); }
+
 bool same_class(const mfm_params& a, const mfm_params& b) {
   return a.spacedim == b.spacedim && a.nfluids == b.nfluids && a.dedner == b.dedner &&
+         solver(a) == solver(b) &&
          a.time_approx == b.time_approx && a.precision == b.precision &&
          (a.char_reconstruction != 0) == (b.char_reconstruction != 0);
 }
