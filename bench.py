@@ -558,8 +558,8 @@ def main():
                          "(calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the patch table would shard)",
+        print("bench.py measures one GPU: the library has no multi-GPU path "
+              "(a level's patch table would shard by patch, one process per GPU)",
               file=sys.stderr)
         sys.exit(2)
 
