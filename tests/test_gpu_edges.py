@@ -1,0 +1,68 @@
+"""Edge cases of the device update through the C ABI: empty and oversized
+inputs are refused on the host before any device work, and degenerate
+patches (one cell, one-cell-thick slabs, a table mixing them with a large
+patch) match the oracle / their single-patch updates."""
+import numpy as np
+import pytest
+
+import mfmhd
+from helpers import GpuPatch, gpu_update, grid_shape, periodic_fill, smooth_state
+from test_gpu_parity import check64, periodic_params, run_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def test_empty_table_refused():
+    ctx = mfmhd.Context(periodic_params(nf=1))
+    with pytest.raises(mfmhd.MfmError) as e:
+        ctx.update_level([], 0.01, 0.05)
+    assert e.value.code == mfmhd.ERR_ARG
+    ctx.close()
+
+
+def test_oversized_fab_refused_before_device_work():
+    """A FAB component of 2^31 cells or more is refused by the host checks
+    (its data pointer is never used)."""
+    p = periodic_params(nf=1)
+    ctx = mfmhd.Context(p)
+    n, ng = 1300, mfmhd.num_ghost_cells(p)
+    fab = mfmhd.fab(0x1000, (-ng,) * 3, (n - 1 + ng,) * 3, 8)
+    with pytest.raises(mfmhd.MfmError) as e:
+        ctx.update_level([(fab, mfmhd.Box.of((0, 0, 0), (n - 1,) * 3))], 0.01, 0.05)
+    assert e.value.code == mfmhd.ERR_SHAPE
+    ctx.close()
+
+
+@pytest.mark.parametrize("vhi", [(0, 0, 0), (0, 11, 9), (9, 0, 11), (11, 9, 0), (13, 0, 0)])
+@pytest.mark.parametrize("time", [mfmhd.TIME_RK2, mfmhd.TIME_HANCOCK])
+def test_degenerate_patches(oracle_mod, vhi, time):
+    print("degenerate patch", vhi, time, flush=True)
+    p = periodic_params(nf=2, flux=mfmhd.FLUX_ROE8, lim=mfmhd.LIM_VANLEER, time=time)
+    ng = mfmhd.num_ghost_cells(p)
+    nc = mfmhd.num_components(p)
+    vlo = (0, 0, 0)
+    lo, hi, shape = grid_shape(nc, vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=sum(vhi) + time)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.005, 0.05)
+    check64(g, o, lo, vlo, vhi, U_in=U)
+
+
+def test_table_mixing_tiny_and_large_patches():
+    p = periodic_params(nf=4, flux=mfmhd.FLUX_ROE8, lim=mfmhd.LIM_MINMOD)
+    ng = mfmhd.num_ghost_cells(p)
+    nc = mfmhd.num_components(p)
+    boxes = [((0, 0, 0), (0, 0, 0)), ((5, 5, 5), (36, 27, 20)), ((-3, 7, 2), (-3, 7, 9)),
+             ((40, 0, 0), (41, 1, 0))]
+    states = []
+    for q, (vlo, vhi) in enumerate(boxes):
+        lo, hi, shape = grid_shape(nc, vlo, vhi, ng)
+        states.append((smooth_state(p, shape, lo, seed=70 + q), lo, vlo, vhi))
+    ctx = mfmhd.Context(p)
+    singles = [gpu_update(ctx, U, lo, vlo, vhi, 0.004, 0.05) for U, lo, vlo, vhi in states]
+    pats = [GpuPatch(U, lo) for U, lo, _, _ in states]
+    ctx.update_level([(g.fab, mfmhd.Box.of(s[2], s[3])) for g, s in zip(pats, states)],
+                     0.004, 0.05)
+    mfmhd.synchronize()
+    for q, (g, s) in enumerate(zip(pats, singles)):
+        assert np.array_equal(g.numpy(), s["U"]), q
+    ctx.close()
