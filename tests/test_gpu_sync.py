@@ -83,16 +83,25 @@ def _state(box, nc, dx, ng):
     return U
 
 
-def test_amr_cycle_conserves_mass():
+@pytest.mark.parametrize("time,char", [(mfmhd.TIME_RK2, 0), (mfmhd.TIME_HANCOCK, 1)])
+def test_amr_cycle_conserves_mass(time, char):
     """Coarse 16^3 periodic level of 8 boxes, fine level (ratio 2) of three
-    boxes, one wrapping around the domain; RK2 single-fluid MHD.  One coarse
-    step and two fine steps with the device ghost fill, then reflux and the
-    average down: the composite mass is unchanged to round-off."""
+    boxes, one wrapping around the domain; single-fluid MHD with RK2, and with
+    the Orszag-Tang deck's scheme (Hancock, characteristic slopes with
+    check_tvd, van Leer).  One coarse step and two fine steps with the device
+    ghost fill, then reflux and the average down: the composite mass is
+    unchanged to round-off."""
+    print("amr cycle", time, char, flush=True)
     ng = 4
     pc = level_params((16, 16, 16), 3, (1, 1, 1))
     pf = level_params((32, 32, 32), 3, (1, 1, 1))
     for p in (pc, pf):
-        p.time_approx = mfmhd.TIME_RK2
+        p.time_approx = time
+        p.char_reconstruction = char
+        if char:
+            p.check_tvd = 1
+            for v in range(8):
+                p.limiter[v] = mfmhd.LIM_VANLEER
     nc = mfmhd.num_components(pc)
     dxc, dxf = 1.0 / 16, 1.0 / 32
     cctx, fctx = mfmhd.Context(pc), mfmhd.Context(pf)
