@@ -66,3 +66,4 @@ def test_table_mixing_tiny_and_large_patches():
     for q, (g, s) in enumerate(zip(pats, singles)):
         assert np.array_equal(g.numpy(), s["U"]), q
     ctx.close()
+
