@@ -128,3 +128,21 @@ def test_ctx_validation_without_device():
     if mfmhd.device_count() == 0:
         p = mfmhd.default_params()
         assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_DEVICE
+
+
+def test_binding_constants_match_the_header():
+    """Every MFM_* enumerator of include/mfm.h has the same value in the
+    Python binding (MFM_LIM_MINMOD -> mfmhd.LIM_MINMOD, MFM_KT_* ->
+    KT_*, MFM_REQ_* -> REQ_*, MFM_OK / MFM_ERR_* -> OK / ERR_*)."""
+    src = open(os.path.join(ROOT, "include", "mfm.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    found = 0
+    for body in re.findall(r"enum\s*\{(.*?)\}", src, flags=re.S):
+        for name, val in re.findall(r"\b(MFM_[A-Z0-9_]+)\s*=\s*(-?\d+)", body):
+            py = name[len("MFM_"):]
+            assert hasattr(mfmhd, py), name
+            assert getattr(mfmhd, py) == int(val), (name, getattr(mfmhd, py), val)
+            found += 1
+    assert found >= 40
+    abi = re.search(r"#define MFM_ABI_VERSION (\d+)", src)
+    assert int(abi.group(1)) == mfmhd.ABI_VERSION
