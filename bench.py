@@ -510,6 +510,60 @@ def orszag_bench(nxy=512, nz=32, steps=10, warmup=3):
     return out
 
 
+def l1_per_variable(a, b, nf):
+    """Relative L1 error of a against b per variable, vectors (momentum,
+    field) scaled by the L1 norm of their whole group."""
+    groups = [[0], [1, 2, 3], [1, 2, 3], [1, 2, 3], [4], [5, 6, 7], [5, 6, 7], [5, 6, 7]]
+    for f in range(1, nf):
+        s0 = 8 + 5 * (f - 1)
+        g = [s0 + 1, s0 + 2, s0 + 3]
+        groups += [[s0], g, g, g, [s0 + 4]]
+    norms = {}
+    out = []
+    for c, g in enumerate(groups):
+        key = tuple(g)
+        if key not in norms:
+            norms[key] = sum(float(np.abs(b[q]).sum()) for q in g)
+        out.append(float(np.abs(a[c] - b[c]).sum()) / max(norms[key], 1e-300))
+    return out
+
+
+def fp64_bench(n, dts, warmup, walk, final32):
+    """The same workload in the FP64 build with the FP32 run's dt sequence:
+    time per step over the steps after the warm-up, and the FP32-flux
+    run's relative L1 error per variable against it at the end (the
+    heliospheric configuration's FP64 run beside the FP32 flux run)."""
+    p, dx = c3_params(n, mfmhd.PREC_FP64)
+    ctx = mfmhd.Context(p)
+    ctx.set_walk(walk[0], walk[1])
+    nc, ng = ctx.ncomp, ctx.nghost
+    lo, hi = (-ng,) * 3, (n - 1 + ng,) * 3
+    m = n + 2 * ng
+    U = mfmhd.DeviceArray((nc, m, m, m))
+    Uf = mfmhd.fab(U.ptr, lo, hi, nc)
+    ctx.init_problem(Uf, mfmhd.Box.of(lo, hi), dx)
+    valid = mfmhd.Box.of((0, 0, 0), (n - 1,) * 3)
+    t = 0.0
+    e0, e1 = mfmhd.Event(), mfmhd.Event()
+    for q, dt in enumerate(dts):
+        if q == warmup:
+            mfmhd.synchronize()
+            e0.record()
+        finite_dt(ctx.update_state(Uf, valid, dt, dx, t)[0], "fp64 update")
+        t += dt
+    e1.record()
+    mfmhd.synchronize()
+    steps = len(dts) - warmup
+    ms = e0.elapsed_ms(e1) / steps
+    final64 = U.numpy()[:, ng:ng + n, ng:ng + n, ng:ng + n]
+    ctx.close()
+    U.free()
+    err = l1_per_variable(final32, final64, p.nfluids)
+    return {"ms_per_step": ms, "cell_updates_per_s": float(n) ** 3 / (ms * 1e-3),
+            "steps_compared": len(dts),
+            "fp32_flux_l1_per_variable": err, "fp32_flux_l1_max": max(err)}
+
+
 def copy_ceiling(gib=4, reps=5):
     """Device-to-device copy bandwidth (read + write bytes per second): the
     HBM rate a streaming kernel can reach on this device."""
@@ -549,6 +603,8 @@ def main():
                     help="skip the batched AMR patch-table measurement")
     ap.add_argument("--no-hancock", action="store_true",
                     help="skip the Hancock-scheme measurement of the same workload")
+    ap.add_argument("--no-fp64", action="store_true",
+                    help="skip the FP64 run of the same workload (and the FP32 error)")
     ap.add_argument("--no-orszag", action="store_true",
                     help="skip the Orszag-Tang deck scheme (characteristic slopes) timing")
     ap.add_argument("--no-fill", action="store_true",
@@ -586,8 +642,10 @@ def main():
         # first step initial_cfl * dt_raw, then growth-limited cfl * dt_raw
         return CFL0 * dtn if dt is None else min(args.cfl * dtn, DT_GROWTH * dt)
 
+    dts = []
     for _ in range(args.warmup):
         dt = next_dt(dtn, dt)
+        dts.append(dt)
         dtn = finite_dt(ctx.update_state(Uf, valid, dt, dx, t)[0], "update")
         t += dt
     mfmhd.synchronize()
@@ -598,6 +656,7 @@ def main():
     w0 = time.perf_counter()
     for _ in range(args.steps):
         dt = next_dt(dtn, dt)
+        dts.append(dt)
         dtn = finite_dt(ctx.update_state(Uf, valid, dt, dx, t)[0], "update")
         t += dt
     e1.record()
@@ -661,7 +720,13 @@ def main():
         "wall_s_timed": wall,
     }
     ctx.close()
+    final32 = None
+    if prec == mfmhd.PREC_FP32_FLUX and not args.no_fp64:
+        final32 = U.numpy()[:, ng:ng + n, ng:ng + n, ng:ng + n].copy()
     U.free()
+    if final32 is not None:
+        out["fp64"] = fp64_bench(n, dts, args.warmup, walk, final32)
+        del final32
     if args.calib_copy > 0:
         nb = args.calib_copy << 20
         a = mfmhd.DeviceArray((nb // 8,), zero=True)
