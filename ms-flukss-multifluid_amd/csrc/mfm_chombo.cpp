@@ -203,6 +203,7 @@ struct File {
           }
           mem.type = std::make_shared<Datatype>();
           r += datatype(r, end, *mem.type, depth + 1);
+          if (mem.offset + mem.type->size > t.size) bad("compound member beyond its type");
           t.members.push_back(mem);
         }
         return (size_t)(r - q);
@@ -532,6 +533,7 @@ bool has_member(const Datatype& t, const char* name) { return member(t, name) !=
 
 // Chombo box compound (lo_i, lo_j[, lo_k], hi_i, hi_j[, hi_k])
 mfm_box box_of(const uint8_t* q, const Datatype& t) {
+  // callers hold at least t.size bytes at q
   static const char* lo[3] = {"lo_i", "lo_j", "lo_k"};
   static const char* hi[3] = {"hi_i", "hi_j", "hi_k"};
   mfm_box b;
@@ -550,6 +552,12 @@ const Attribute& need_attr(File& f, const Object& o, const std::string& name) {
   const Attribute* a = f.attribute(o, name);
   if (!a) throw H5Error(MFM_ERR_ARG, "missing attribute " + name);
   return *a;
+}
+// First element of an attribute, checked to be present.
+const uint8_t* first(const Attribute& a) {
+  if (a.type.size == 0 || a.data.size() < a.type.size)
+    throw H5Error(MFM_ERR_ARG, "attribute " + a.name + " holds no value");
+  return a.data.data();
 }
 std::string attr_string(const Attribute& a) {
   if (a.type.cls != 3) throw H5Error(MFM_ERR_ARG, "attribute " + a.name + " is not a string");
@@ -640,16 +648,16 @@ int mfm_chombo_open(const char* path, mfm_chombo_file** out) {
     c->f.open(path);
     const Object& root = c->f.object(c->f.root);
     const Attribute& nl = need_attr(c->f, root, "num_levels");
-    c->num_levels = (int)as_int(nl.data.data(), nl.type);
+    c->num_levels = (int)as_int(first(nl), nl.type);
     const Attribute& ncomp = need_attr(c->f, root, "num_components");
-    c->num_components = (int)as_int(ncomp.data.data(), ncomp.type);
-    if (const Attribute* t = c->f.attribute(root, "time")) c->time = as_real(t->data.data(), t->type);
+    c->num_components = (int)as_int(first(ncomp), ncomp.type);
+    if (const Attribute* t = c->f.attribute(root, "time")) c->time = as_real(first(*t), t->type);
     for (int k = 0; k < c->num_components; ++k)
       c->names.push_back(attr_string(need_attr(c->f, root, "component_" + std::to_string(k))));
     try {
       const Object& g = c->f.object(c->f.resolve("/Chombo_global"));
       if (const Attribute* s = c->f.attribute(g, "SpaceDim"))
-        c->space_dim = (int)as_int(s->data.data(), s->type);
+        c->space_dim = (int)as_int(first(*s), s->type);
     } catch (const H5Error&) {
     }
     if (c->num_levels < 1 || c->num_components < 1) throw H5Error(MFM_ERR_ARG, "empty file");
@@ -690,19 +698,19 @@ int mfm_chombo_level(mfm_chombo_file* c, int level, int* nboxes, double* dx, dou
     if (level < 0 || level >= c->num_levels) throw H5Error(MFM_ERR_ARG, "level out of range");
     std::string g = "/level_" + std::to_string(level);
     const Object& lo = c->f.object(c->f.resolve(g));
-    if (dx) { const Attribute& a = need_attr(c->f, lo, "dx"); *dx = as_real(a.data.data(), a.type); }
+    if (dx) { const Attribute& a = need_attr(c->f, lo, "dx"); *dx = as_real(first(a), a.type); }
     if (dt) {
       const Attribute* a = c->f.attribute(lo, "dt");
-      *dt = a ? as_real(a->data.data(), a->type) : 0.0;
+      *dt = a ? as_real(first(*a), a->type) : 0.0;
     }
     if (ref_ratio) {
       const Attribute* a = c->f.attribute(lo, "ref_ratio");
-      *ref_ratio = a ? (int)as_int(a->data.data(), a->type) : 1;
+      *ref_ratio = a ? (int)as_int(first(*a), a->type) : 1;
     }
     if (domain) {
       const Attribute& a = need_attr(c->f, lo, "prob_domain");
       if (a.type.cls != 6) throw H5Error(MFM_ERR_ARG, "prob_domain is not a box");
-      *domain = box_of(a.data.data(), a.type);
+      *domain = box_of(first(a), a.type);
     }
     const Object& bx = c->f.object(c->f.resolve(g + "/boxes"));
     if (bx.type.cls != 6 || bx.dims.size() != 1) throw H5Error(MFM_ERR_ARG, "boxes is not a box list");
@@ -716,7 +724,7 @@ int mfm_chombo_level(mfm_chombo_file* c, int level, int* nboxes, double* dx, dou
         if (a) {
           static const char* nm[3] = {"intvecti", "intvectj", "intvectk"};
           for (int d = 0; d < 3; ++d)
-            if (has_member(a->type, nm[d])) ghost[d] = member_int(a->data.data(), a->type, nm[d]);
+            if (has_member(a->type, nm[d])) ghost[d] = member_int(first(*a), a->type, nm[d]);
         }
       } catch (const H5Error& e) {
         if (e.code != MFM_ERR_ARG) throw;
@@ -733,6 +741,7 @@ int mfm_chombo_boxes(mfm_chombo_file* c, int level, mfm_box* boxes, int capacity
     int nb = (int)bx.dims[0];
     if (!boxes || capacity < nb) throw H5Error(MFM_ERR_SHAPE, "box buffer too small");
     auto r = c->f.raw(bx);
+    if ((uint64_t)nb * bx.type.size > r.second) throw H5Error(MFM_ERR_ARG, "short box list");
     for (int b = 0; b < nb; ++b) boxes[b] = box_of(r.first + (size_t)b * bx.type.size, bx.type);
   });
 }
@@ -747,11 +756,13 @@ int mfm_chombo_read_box(mfm_chombo_file* c, int level, int b, double* dst, size_
     if (off.dims.size() != 1 || dat.dims.size() != 1) throw H5Error(MFM_ERR_ARG, "bad data layout");
     if (b < 0 || (uint64_t)b + 1 >= off.dims[0]) throw H5Error(MFM_ERR_ARG, "box out of range");
     auto ro = c->f.raw(off);
+    if ((uint64_t)(b + 2) * off.type.size > ro.second) throw H5Error(MFM_ERR_ARG, "short offsets");
     int64_t o0 = as_int(ro.first + (size_t)b * off.type.size, off.type);
     int64_t o1 = as_int(ro.first + (size_t)(b + 1) * off.type.size, off.type);
     if (o0 < 0 || o1 < o0 || (uint64_t)o1 > dat.dims[0]) throw H5Error(MFM_ERR_ARG, "bad offsets");
     if (!dst || count != (size_t)(o1 - o0)) throw H5Error(MFM_ERR_SHAPE, "value count differs from the box");
     auto rd = c->f.raw(dat);
+    if ((uint64_t)o1 * dat.type.size > rd.second) throw H5Error(MFM_ERR_ARG, "short data");
     const uint8_t* q = rd.first + (size_t)o0 * dat.type.size;
     if (dat.type.cls == 1 && dat.type.size == 8 && !dat.type.big_endian) {
       std::memcpy(dst, q, count * 8);
