@@ -564,6 +564,23 @@ def fp64_bench(n, dts, warmup, walk, final32):
             "fp32_flux_l1_per_variable": err, "fp32_flux_l1_max": max(err)}
 
 
+def host_path_bench(ctx, U, lo, valid, dt, dx, t, reps=3):
+    """mfm_patch_update_host on a host copy of the state (pageable memory):
+    each call stages U host -> HBM -> host around one update, so its rate
+    includes PCIe both ways.  Wall time per call, best of `reps`."""
+    Uh = U.numpy()
+    walls = []
+    for _ in range(reps):
+        w0 = time.perf_counter()
+        finite_dt(ctx.update_state_host(Uh, lo, valid, dt, dx, t)[0], "host update")
+        walls.append(time.perf_counter() - w0)
+    ms = 1e3 * min(walls)
+    cells = float(np.prod(valid.shape()))
+    return {"ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3),
+            "staged_bytes_per_step": 2 * Uh.nbytes, "calls": reps,
+            "memory": "pageable host FAB"}
+
+
 def copy_ceiling(gib=4, reps=5):
     """Device-to-device copy bandwidth (read + write bytes per second): the
     HBM rate a streaming kernel can reach on this device."""
@@ -607,6 +624,8 @@ def main():
                     help="skip the FP64 run of the same workload (and the FP32 error)")
     ap.add_argument("--no-orszag", action="store_true",
                     help="skip the Orszag-Tang deck scheme (characteristic slopes) timing")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-buffer (PCIe-inclusive) update measurement")
     ap.add_argument("--no-fill", action="store_true",
                     help="skip the level ghost-fill measurement")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -719,6 +738,8 @@ def main():
         "kernels": kern,
         "wall_s_timed": wall,
     }
+    if not args.no_host:
+        out["host_path"] = host_path_bench(ctx, U, lo, valid, dt, dx, t)
     ctx.close()
     final32 = None
     if prec == mfmhd.PREC_FP32_FLUX and not args.no_fp64:

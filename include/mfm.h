@@ -90,7 +90,11 @@ typedef struct mfm_params {
   int flux;                 /* MFM_FLUX_*                                   */
   int limiter[8];           /* limiter_RHO..limiter_BZ; all other slopes   */
                             /* (neutrals, psi) use minmod                  */
-  int check_positivity;     /* mhdam.check_positivity                       */
+  int check_positivity;     /* mhdam.check_positivity; accepted, no effect:
+                               in the reference it only queries the slopes'
+                               flag, which stays false (main/MHDAM.cpp:978-987,
+                               src/slopes.cpp:84-87), so its positivity
+                               fallback never runs                           */
   int time_approx;          /* MFM_TIME_FIRST_ORDER, _HANCOCK or _RK2       */
   int divb_method;          /* mhdam.divB_method: 0 none, 1 faces, 2 CD     */
   int eight_wave;           /* mhdam.8wave_use                              */

@@ -1007,6 +1007,14 @@ int mfm_patch_update_host(mfm_ctx* c, mfm_fab U, mfm_box valid, double dt,
   if (!c || !U.data) return MFM_ERR_ARG;
   const int dims = c->C.dims;
   HBox ub = hb(U.box);
+  // sizes of the staged copies come from the host FABs: refuse empty or
+  // negative ones before any allocation (the device path checks the rest)
+  if (U.ncomp <= 0 || ub.size() == 0) return fail(c, MFM_ERR_SHAPE, "U: empty FAB");
+  for (int d = 0; F && d < dims; ++d)
+    if (!F[d].data || F[d].ncomp <= 0 || hb(F[d].box).size() == 0)
+      return fail(c, MFM_ERR_SHAPE, "F[d]: empty FAB");
+  if (divB && (!divB->data || divB->ncomp <= 0 || hb(divB->box).size() == 0))
+    return fail(c, MFM_ERR_SHAPE, "divB: empty FAB");
   size_t ub_bytes = (size_t)ub.size() * U.ncomp * sizeof(double);
   double* dU = nullptr;
   std::vector<double*> dF(3, nullptr);
@@ -1021,7 +1029,9 @@ int mfm_patch_update_host(mfm_ctx* c, mfm_fab U, mfm_box valid, double dt,
     for (int d = 0; d < dims && e == hipSuccess; ++d) {
       size_t b = (size_t)hb(F[d].box).size() * F[d].ncomp * sizeof(double);
       e = hipMalloc((void**)&dF[d], b);
-      if (e == hipSuccess) e = hipMemset(dF[d], 0, b);
+      // the caller's values travel both ways, so cells of F outside the
+      // faces of `valid` come back unchanged, as with device FABs
+      if (e == hipSuccess) e = hipMemcpy(dF[d], F[d].data, b, hipMemcpyHostToDevice);
       Fd[d] = F[d];
       Fd[d].data = dF[d];
     }

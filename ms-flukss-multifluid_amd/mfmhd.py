@@ -411,6 +411,33 @@ class Context:
             return nd.value, tuple(cell)
         return rc, nd.value, tuple(cell)
 
+    def update_state_host(self, U, lo, valid, dt, dx, time=0.0, F=None, divB=None):
+        """mfm_patch_update_host: the same update on host arrays (C-ordered
+        float64 (nc, nz, ny, nx) FAB data whose low corner is `lo`), staged
+        through HBM by the library; U (and F, divB) are written back in
+        place.  F: one host array per direction, (nc, nz, ny, nx) of the face
+        box, with its low corner; divB likewise -> (new_dt, cell)."""
+        import numpy as np
+
+        def host_fab(a, alo):
+            if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.ndim == 4
+                    and a.flags.c_contiguous and a.flags.writeable):
+                raise MfmError(ERR_ARG, "host FAB must be a writeable C-ordered float64 "
+                                        "(nc, nz, ny, nx) array")
+            ahi = tuple(alo[d] + a.shape[3 - d] - 1 for d in range(3))
+            return fab(a.ctypes.data, alo, ahi, a.shape[0])
+        Uf = host_fab(U, lo)
+        nd = C.c_double()
+        cell = (C.c_int * 3)()
+        Fp = None
+        if F is not None:
+            arr = (Fab * 3)(*[host_fab(a, alo) for a, alo in F])
+            Fp = C.cast(arr, C.POINTER(Fab))
+        dp = C.pointer(host_fab(*divB)) if divB is not None else None
+        self._check(self.lib.mfm_patch_update_host(self.h, Uf, valid, dt, dx, time, Fp, dp,
+                                                   C.byref(nd), cell))
+        return nd.value, tuple(cell)
+
     def update_level(self, patches, dt, dx, time=0.0, stream=None):
         """One launch sequence over a patch table (list of (U, valid) or
         (U, valid, F) with F the patch's face-flux FABs, one per direction)

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import mfmhd
-from helpers import GpuPatch, gpu_update, grid_shape, periodic_fill, smooth_state
+from helpers import GpuPatch, gpu_update, grid_shape, periodic_fill, smooth_state, valid_view
 from test_gpu_parity import check64, periodic_params, run_pair
 
 pytestmark = pytest.mark.gpu
@@ -67,3 +67,46 @@ def test_table_mixing_tiny_and_large_patches():
         assert np.array_equal(g.numpy(), s["U"]), q
     ctx.close()
 
+
+
+def trough_state(p, n, ng):
+    """A periodic state whose plasma is scaled by a deep, steep trough along
+    x (0.01, 0.2, 3.0 at x = 5, 6, 7; pressure scales with it): hyperbee
+    slopes then give non-positive face densities."""
+    nc = mfmhd.num_components(p)
+    vlo, vhi = (0, 0, 0), (n - 1,) * 3
+    lo, hi, shape = grid_shape(nc, vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=3)
+    f = np.ones(n)
+    f[5], f[6], f[7] = 0.01, 0.2, 3.0
+    V = U[:, ng:ng + n, ng:ng + n, ng:ng + n]
+    b2 = (V[5] ** 2 + V[6] ** 2 + V[7] ** 2) / (8 * np.pi)
+    V[0:4] *= f
+    V[4] = f * (V[4] - b2) + b2
+    periodic_fill(U, ng)
+    return U, lo, vlo, vhi
+
+
+@pytest.mark.parametrize("time", [mfmhd.TIME_RK2, mfmhd.TIME_HANCOCK])
+def test_check_positivity_has_no_effect(oracle_mod, time):
+    """mhdam.check_positivity is inert in the reference (main/MHDAM.cpp:978-987
+    only queries slopes::checkPositivity(), src/slopes.cpp:84-87, and the flag
+    stays false), so REDUCEORDER never runs there.  On a state with
+    non-positive face densities the device update is the same bit for bit
+    with the option on or off, and matches the oracle."""
+    print("check_positivity", time, flush=True)
+    p = periodic_params(nf=2, flux=mfmhd.FLUX_ROE8, lim=mfmhd.LIM_HYPERBEE, time=time)
+    ng, n, dx = mfmhd.num_ghost_cells(p), 12, 0.05
+    U, lo, vlo, vhi = trough_state(p, n, ng)
+    WM, WP = oracle_mod.face_values(p, U, lo, 0, (-1, -1, -1), (n, n, n))
+    faces = np.concatenate([valid_view(W, lo, (-1,) * 3, (n,) * 3)[0].ravel() for W in (WM, WP)])
+    assert (faces <= 0).sum() > 0
+    dt, _ = oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)
+    out = []
+    for cp in (0, 1):
+        p.check_positivity = cp
+        g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.3 * dt, dx)
+        check64(g, o, lo, vlo, vhi, U_in=U)
+        out.append(g)
+    assert np.array_equal(out[0]["U"], out[1]["U"])
+    assert out[0]["dt"] == out[1]["dt"] and out[0]["cell"] == out[1]["cell"]
