@@ -1079,6 +1079,8 @@ int mfm_init_problem_dev(mfm_ctx* c, mfm_fab U, mfm_box box, double dx,
   if (U.ncomp != c->C.nc) return fail(c, MFM_ERR_SHAPE, "U.ncomp mismatch");
   HBox ub = hb(U.box), b = hb(box);
   if (!contains(ub, b)) return fail(c, MFM_ERR_SHAPE, "box outside U");
+  if (ub.size() >= (1L << 31)) return fail(c, MFM_ERR_SHAPE, "FAB of 2^31 cells or more");
+  if (b.size() == 0) return MFM_OK;  // an empty box: nothing to set
   int ulo[3], un[3], lo[3], n[3];
   for (int d = 0; d < 3; ++d) { ulo[d] = ub.lo[d]; un[d] = (int)ub.n(d); lo[d] = b.lo[d]; n[d] = (int)b.n(d); }
   hipError_t e = launch_init(c->C, U.data, ulo, un, ub.size(), lo, n, dx, (hipStream_t)stream);
@@ -1089,7 +1091,10 @@ int mfm_init_problem_dev(mfm_ctx* c, mfm_fab U, mfm_box box, double dx,
 int mfm_fill_periodic_dev(mfm_ctx* c, mfm_fab U, mfm_box valid, void* stream) {
   if (!c || !U.data) return MFM_ERR_ARG;
   HBox ub = hb(U.box), v = hb(valid);
+  // the kernel takes every index modulo the valid extent: it must be >= 1
+  if (v.size() == 0 || U.ncomp <= 0) return fail(c, MFM_ERR_SHAPE, "empty valid box or FAB");
   if (!contains(ub, v)) return fail(c, MFM_ERR_SHAPE, "valid outside U");
+  if (ub.size() >= (1L << 31)) return fail(c, MFM_ERR_SHAPE, "FAB of 2^31 cells or more");
   int ulo[3], un[3], vlo[3], vn[3];
   for (int d = 0; d < 3; ++d) { ulo[d] = ub.lo[d]; un[d] = (int)ub.n(d); vlo[d] = v.lo[d]; vn[d] = (int)v.n(d); }
   hipError_t e = launch_fill_periodic(U.data, ulo, un, ub.size(), U.ncomp, vlo, vn,
