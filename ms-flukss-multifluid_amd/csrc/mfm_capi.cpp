@@ -603,6 +603,9 @@ int run_dt(mfm_ctx* c, const Consts& C0, const mfm_fab& U, const mfm_box& box,
   if (!c || !U.data || !dt) return MFM_ERR_ARG;
   if (U.ncomp != C0.nc) return fail(c, MFM_ERR_SHAPE, "U.ncomp mismatch");
   HBox ub = hb(U.box), b = hb(box);
+  // as the update: an empty box has no cell to reduce over (and would leave
+  // the region list below without its one region)
+  if (b.size() == 0) return fail(c, MFM_ERR_SHAPE, "empty box");
   if (!contains(ub, b)) return fail(c, MFM_ERR_SHAPE, "box outside U");
   if (ub.size() >= (1L << 31)) return fail(c, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^31 cells");
   if (!(dx > 0.0)) return fail(c, MFM_ERR_ARG, "dx must be positive");

@@ -1,7 +1,8 @@
 """Runtime entry points of the C ABI beside the update: the periodic ghost
 fill of a one-box level (mfm_fill_periodic_dev), per-kernel timing
-(mfm_ctx_set_timing / mfm_ctx_get_timing), the work walk setters and the
-event helpers.  None of them may change the update's results."""
+(mfm_ctx_set_timing / mfm_ctx_get_timing), the work walk setters, the
+event helpers, and the box checks of mfm_patch_compute_dt_dev.  None of them
+may change the update's results."""
 import numpy as np
 import pytest
 
@@ -114,4 +115,19 @@ def test_walk_setters_and_events():
     e1.record()
     mfmhd.synchronize()
     assert e0.elapsed_ms(e1) > 0.0
+    ctx.close()
+
+
+def test_compute_dt_refuses_empty_or_outside_box():
+    """Refused on the host before any device work, like the update."""
+    p = periodic_params(nf=1)
+    ctx = mfmhd.Context(p)
+    U = smooth_state(p, (8, 6, 6, 6), (0, 0, 0), seed=1)
+    g = GpuPatch(U, (0, 0, 0))
+    for vlo, vhi in (((2, 2, 2), (1, 3, 3)), ((1, 1, 1), (6, 4, 4))):
+        with pytest.raises(mfmhd.MfmError) as e:
+            ctx.compute_dt(g.fab, mfmhd.Box.of(vlo, vhi), 0.05)
+        assert e.value.code == mfmhd.ERR_SHAPE
+    dt, cell = ctx.compute_dt(g.fab, mfmhd.Box.of((1, 1, 1), (4, 4, 4)), 0.05)
+    assert dt > 0.0 and all(1 <= c <= 4 for c in cell)
     ctx.close()
