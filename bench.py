@@ -50,7 +50,7 @@ def finite_dt(dtn, where):
 # with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the default workload
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 from a 1 GiB copy; summary in
 # profiles/k_update_helio256_traffic.md).  Valid for that workload only.
-TRAFFIC = {(256, "fp32flux"): 15141000000.0}
+TRAFFIC = {(256, "fp32flux"): 14241000000.0}
 
 
 def c3_params(n, prec):
