@@ -272,6 +272,35 @@ def test_fp32_flux_path_error(oracle_mod, nf):
     assert err.max() <= TOL32_L1, err
 
 
+@pytest.mark.parametrize("flux,dedner", [(mfmhd.FLUX_RUSANOV, False),
+                                         (mfmhd.FLUX_ROE8_ROEAVG, False),
+                                         (mfmhd.FLUX_RUSANOV_ROEAVG, False),
+                                         (mfmhd.FLUX_ROE8, True)])
+def test_fp32_flux_solvers(oracle_mod, flux, dedner):
+    """The FP32-flux kernels of the other Riemann solver families (Rusanov,
+    Roe-averaged states, 7-wave Roe + GLM): per-variable relative L1 error
+    against the FP64 oracle <= 1e-5, as for the 8-wave Roe solver."""
+    print("fp32 solver", flux, dedner, flush=True)
+    nf = 1 if dedner else 2
+    p = periodic_params(nf=nf, flux=flux, lim=mfmhd.LIM_MINMOD, dedner=dedner,
+                        eight=not dedner, prec=mfmhd.PREC_FP32_FLUX)
+    ng = mfmhd.num_ghost_cells(p)
+    vlo, vhi = (0, 0, 0), (15, 15, 15)
+    lo, hi, shape = grid_shape(mfmhd.num_components(p), vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=40 + flux)
+    periodic_fill(U, ng)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.004, 0.05, ch=2.0, cp=1.5)
+    assert g["rc"] == 0 and o["rc"] == 0
+    a = valid_view(g["U"], lo, vlo, vhi)
+    b = valid_view(o["U"], lo, vlo, vhi)
+    groups = vector_groups(nf, dedner)
+    if dedner:
+        groups.append((5, 6, 7, 8))   # psi on the scale of the field it cleans
+    err = l1_rel_vector_scaled(a, b, groups)
+    print("fp32 per-variable L1:", " ".join("%.2e" % e for e in err))
+    assert err.max() <= TOL32_L1, err
+
+
 def test_shape_errors_leave_state_untouched():
     p = periodic_params(nf=1)
     ctx = mfmhd.Context(p)
