@@ -272,23 +272,28 @@ def test_fp32_flux_path_error(oracle_mod, nf):
     assert err.max() <= TOL32_L1, err
 
 
-@pytest.mark.parametrize("flux,dedner", [(mfmhd.FLUX_RUSANOV, False),
-                                         (mfmhd.FLUX_ROE8_ROEAVG, False),
-                                         (mfmhd.FLUX_RUSANOV_ROEAVG, False),
-                                         (mfmhd.FLUX_ROE8, True)])
-def test_fp32_flux_solvers(oracle_mod, flux, dedner):
+@pytest.mark.parametrize("flux,dedner,time,dims", [
+    (mfmhd.FLUX_RUSANOV, False, mfmhd.TIME_RK2, 3),
+    (mfmhd.FLUX_ROE8_ROEAVG, False, mfmhd.TIME_RK2, 3),
+    (mfmhd.FLUX_RUSANOV_ROEAVG, False, mfmhd.TIME_RK2, 3),
+    (mfmhd.FLUX_ROE8, True, mfmhd.TIME_RK2, 3),
+    (mfmhd.FLUX_ROE8, True, mfmhd.TIME_RK2, 2),
+    (mfmhd.FLUX_RUSANOV, False, mfmhd.TIME_HANCOCK, 3),
+    (mfmhd.FLUX_ROE8_ROEAVG, False, mfmhd.TIME_HANCOCK, 3)])
+def test_fp32_flux_solvers(oracle_mod, flux, dedner, time, dims):
     """The FP32-flux kernels of the other Riemann solver families (Rusanov,
-    Roe-averaged states, 7-wave Roe + GLM): per-variable relative L1 error
-    against the FP64 oracle <= 1e-5, as for the 8-wave Roe solver."""
-    print("fp32 solver", flux, dedner, flush=True)
+    Roe-averaged states, 7-wave Roe + GLM in 3-D and 2-D), RK2 and Hancock:
+    per-variable relative L1 error against the FP64 oracle <= 1e-5, as for
+    the 8-wave Roe solver."""
+    print("fp32 solver", flux, dedner, time, dims, flush=True)
     nf = 1 if dedner else 2
-    p = periodic_params(nf=nf, flux=flux, lim=mfmhd.LIM_MINMOD, dedner=dedner,
-                        eight=not dedner, prec=mfmhd.PREC_FP32_FLUX)
+    p = periodic_params(nf=nf, flux=flux, lim=mfmhd.LIM_MINMOD, dims=dims, dedner=dedner,
+                        eight=not dedner, time=time, prec=mfmhd.PREC_FP32_FLUX)
     ng = mfmhd.num_ghost_cells(p)
-    vlo, vhi = (0, 0, 0), (15, 15, 15)
-    lo, hi, shape = grid_shape(mfmhd.num_components(p), vlo, vhi, ng)
-    U = smooth_state(p, shape, lo, seed=40 + flux)
-    periodic_fill(U, ng)
+    vlo, vhi = (0, 0, 0), (15, 15, 15 if dims == 3 else 0)
+    lo, hi, shape = grid_shape(mfmhd.num_components(p), vlo, vhi, ng, dims)
+    U = smooth_state(p, shape, lo, seed=40 + flux + time)
+    periodic_fill(U, ng, dims)
     g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.004, 0.05, ch=2.0, cp=1.5)
     assert g["rc"] == 0 and o["rc"] == 0
     a = valid_view(g["U"], lo, vlo, vhi)
