@@ -96,7 +96,10 @@ typedef struct mfm_params {
                                src/slopes.cpp:84-87), so its positivity
                                fallback never runs                           */
   int time_approx;          /* MFM_TIME_FIRST_ORDER, _HANCOCK or _RK2       */
-  int divb_method;          /* mhdam.divB_method: 0 none, 1 faces, 2 CD     */
+  int divb_method;          /* mhdam.divB_method: 1 faces (COMPUTEBN), 2 CD
+                               (COMPUTEBNCD).  0 is refused: the reference
+                               then leaves the face field unset
+                               (srcIdeal/PatchMHDMF.cpp:517) and reads it  */
   int eight_wave;           /* mhdam.8wave_use                              */
   int coord_sys;            /* MFM_COORD_*                                  */
   int precision;            /* MFM_PREC_*                                   */

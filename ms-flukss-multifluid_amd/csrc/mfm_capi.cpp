@@ -135,6 +135,14 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
       }
   if (p->precision != MFM_PREC_FP64 && p->precision != MFM_PREC_FP32_FLUX) { why = "bad precision"; return MFM_ERR_ARG; }
   if (p->divb_method < 0 || p->divb_method > 2) { why = "bad divB_method"; return MFM_ERR_ARG; }
+  if (p->divb_method == 0) {
+    // PatchMHDMF::computeBn returns at once for method 0
+    // (srcIdeal/PatchMHDMF.cpp:517), leaving the face-field FluxBox of
+    // EulerStep (src/PatchMHDAM.cpp:547) unset; the 8-wave source, fluxBC and
+    // the divB output then read unset memory, so no result can match it
+    why = "divB_method = 0 leaves the face field unset in the reference";
+    return MFM_ERR_UNSUPPORTED;
+  }
   for (int v = 0; v < 8; ++v)
     if (p->limiter[v] < 0 || p->limiter[v] > MFM_LIM_NONE) { why = "bad limiter"; return MFM_ERR_ARG; }
   if (!(p->gamma > 1.0)) { why = "gamma must exceed 1"; return MFM_ERR_ARG; }

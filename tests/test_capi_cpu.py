@@ -116,6 +116,11 @@ def test_ctx_validation_without_device():
     p = mfmhd.default_params()
     p.time_approx = 3
     assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
+    p = mfmhd.default_params()            # divB_method 0: the reference reads an unset field
+    p.divb_method = 0
+    assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
+    p.divb_method = 3
+    assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_ARG
     p = mfmhd.default_params()            # characteristic slopes: FP64 build only
     p.char_reconstruction = 1
     p.precision = mfmhd.PREC_FP32_FLUX

@@ -72,6 +72,7 @@ CASES = [
     ("roe8-venkat-1f", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_VENKAT),
     ("roe8-weno3yc-1f", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_WENO3YC),
     ("roe8-cd-1f", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_CD),
+    ("rusanov-minmod-4f", 4, mfmhd.FLUX_RUSANOV, mfmhd.LIM_MINMOD),
 ]
 
 
@@ -113,6 +114,42 @@ def test_fluxes_and_divb_outputs(oracle_mod):
         err = rel_err_per_comp(g["F"][d], o["F"][d])
         assert err.max() <= TOL64, (d, err)
     assert rel_err_per_comp(g["divB"], o["divB"]).max() <= 1e-11
+
+
+@pytest.mark.parametrize("method", [2])
+@pytest.mark.parametrize("time", [mfmhd.TIME_RK2, mfmhd.TIME_HANCOCK])
+def test_divb_methods(oracle_mod, method, time):
+    """mhdam.divB_method 2 (COMPUTEBNCD, srcIdeal/PatchIdealMHDF.ChF:404-426:
+    the face-normal field from the centred cell values) with 8-wave sources,
+    state and divB output against the oracle (method 1 is the default of
+    every other test; method 0 is refused at context creation)."""
+    print("divb method", method, time, flush=True)
+    p = periodic_params(nf=2, flux=mfmhd.FLUX_ROE8, lim=mfmhd.LIM_VANLEER, time=time)
+    p.divb_method = method
+    ng = mfmhd.num_ghost_cells(p)
+    vlo, vhi = (0, 0, 0), (11, 9, 7)
+    lo, hi, shape = grid_shape(mfmhd.num_components(p), vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=60 + method + time)
+    periodic_fill(U, ng)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.004, 0.05, want_divB=True)
+    check64(g, o, lo, vlo, vhi, U_in=U)
+    assert rel_err_per_comp(g["divB"], o["divB"]).max() <= 1e-11
+
+
+def test_dedner_roe_averaged_3d(oracle_mod):
+    """GLM cleaning with the Roe-averaged 7-wave solver (flux = 3 with
+    divB_cleaning = 5), 3-D, two steps so that psi is non-zero."""
+    p = periodic_params(nf=1, flux=mfmhd.FLUX_ROE8_ROEAVG, lim=mfmhd.LIM_MINMOD,
+                        dedner=True, eight=False)
+    ng = mfmhd.num_ghost_cells(p)
+    vlo, vhi = (0, 0, 0), (11, 11, 11)
+    lo, hi, shape = grid_shape(mfmhd.num_components(p), vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=77)
+    for step in range(2):
+        periodic_fill(U, ng)
+        g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.004, 0.05, ch=2.0, cp=1.5)
+        check64(g, o, lo, vlo, vhi)
+        U = o["U"]
 
 
 def test_orszag_tang_dedner_2d(oracle_mod):
