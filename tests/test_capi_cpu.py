@@ -121,6 +121,9 @@ def test_ctx_validation_without_device():
     assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
     p.divb_method = 3
     assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_ARG
+    p = mfmhd.default_params()            # Cartesian coordinates only
+    p.coord_sys = mfmhd.COORD_AXISYMMETRIC
+    assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
     p = mfmhd.default_params()            # characteristic slopes: FP64 build only
     p.char_reconstruction = 1
     p.precision = mfmhd.PREC_FP32_FLUX

@@ -136,6 +136,24 @@ def test_divb_methods(oracle_mod, method, time):
     assert rel_err_per_comp(g["divB"], o["divB"]).max() <= 1e-11
 
 
+@pytest.mark.parametrize("flux", [mfmhd.FLUX_ROE8, mfmhd.FLUX_ROE8_ROEAVG])
+def test_solver_parameters(oracle_mod, flux):
+    """Non-default entropy_fix_coeff (0.05) and smallB (1.0, so that many
+    faces take the 8-wave solver's degenerate-transverse-field branches)
+    reach the device solver as they reach the oracle (both change the
+    result: by 2.7e-2 from the defaults on this state)."""
+    p = periodic_params(nf=2, flux=flux, lim=mfmhd.LIM_MINMOD)
+    p.entropy_fix = 0.05
+    p.small_b = 1.0
+    ng = mfmhd.num_ghost_cells(p)
+    vlo, vhi = (0, 0, 0), (11, 11, 11)
+    lo, hi, shape = grid_shape(mfmhd.num_components(p), vlo, vhi, ng)
+    U = smooth_state(p, shape, lo, seed=88 + flux)
+    periodic_fill(U, ng)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 0.004, 0.05)
+    check64(g, o, lo, vlo, vhi)
+
+
 def test_dedner_roe_averaged_3d(oracle_mod):
     """GLM cleaning with the Roe-averaged 7-wave solver (flux = 3 with
     divB_cleaning = 5), 3-D, two steps so that psi is non-zero."""
