@@ -244,6 +244,25 @@ int mfm_patch_update_host(mfm_ctx* ctx, mfm_fab U, mfm_box valid,
                           double dt, double dx, double time,
                           mfm_fab* F, mfm_fab* divB,
                           double* new_dt, int min_cell[3]);
+/* Allocation-free variant of mfm_patch_update_dev.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory during
+ * the call (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, results and errors as
+ * mfm_patch_update_dev, but all device scratch of the update (first-stage
+ * state, primitives, face fluxes, tables, CFL partials) is taken from one
+ * fixed array of MFM_NOALLOC_SCRATCH_BYTES per GPU, held from library load
+ * to process end, and the staged tables from the context; the call makes no
+ * device or pinned allocation.  A patch whose scratch exceeds the array is
+ * refused with MFM_ERR_NOMEM before any device work (an RK2 step of a
+ * 32^3 patch with 4 ghosts and 23 components needs about 75 MB in FP64).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other streams, other threads, other contexts) race.  Graph replay
+ * (mfm_ctx_set_graphs) is not used by this entry point.                    */
+#define MFM_NOALLOC_SCRATCH_BYTES (128u * 1024u * 1024u)
+int mfm_patch_update_noalloc_dev(mfm_ctx* ctx, mfm_fab U, mfm_box valid,
+                                 double dt, double dx, double time,
+                                 mfm_fab* F, mfm_fab* divB,
+                                 double* new_dt, int min_cell[3], void* stream);
 /* PatchMHDMF::computeDt over `box` (srcIdeal/PatchMHDMF.cpp:66-146). */
 int mfm_patch_compute_dt_dev(mfm_ctx* ctx, mfm_fab U, mfm_box box, double dx,
                              double* dt, int min_cell[3], void* stream);

@@ -36,6 +36,7 @@ UNITS = [
     ("mfm_sync.hip", ["-ffp-contract=off"]),
     ("mfm_fill_plan.cpp", ["-ffp-contract=off"]),
     ("mfm_chombo.cpp", []),
+    ("mfm_noalloc.hip", ["-ffp-contract=off"]),
 ]
 
 

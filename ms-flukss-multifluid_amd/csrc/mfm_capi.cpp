@@ -84,7 +84,7 @@ HBox clip(const Consts& C, HBox b) {
 }
 
 struct RegionBuilder {
-  std::vector<Region> v;
+  SmallVec<Region, 8> v;  // the lists of one patch fit inline
   int nblk = 0;
   long items = 0;
   void add(int patch, const HBox& b) {
@@ -225,7 +225,7 @@ struct PatchGeo {
 // precision).  c provides the scratch arena, staging, timer and errors.
 int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
               double* new_dt, int* min_cell, double* level_dt, int* min_patch,
-              hipStream_t s) {
+              hipStream_t s, const FixedScratch* fixed) {
   const Consts& C0 = Ct[0];
   const mfm_params& p = c->p;
   const int dims = C0.dims;
@@ -236,7 +236,8 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
   for (int r = 1; r < nct; ++r)
     if (Ct[r].dims != C0.dims || Ct[r].nf != C0.nf || Ct[r].dedner != C0.dedner)
       return fail(c, MFM_ERR_ARG, "constants of one batch must share dimension, fluids and GLM");
-  std::vector<PatchGeo> G(n);
+  SmallVec<PatchGeo, 1> G;
+  G.resize(n);
   size_t off = 0;
   const size_t fs = c->fsize();
   for (int q = 0; q < n; ++q) {
@@ -391,14 +392,32 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
   size_t off_idp = off; off += align256(sizeof(long) * nbu);
   size_t off_dto = off; off += align256(sizeof(double) * n);
   size_t off_ido = off; off += align256(sizeof(long) * n);
-  int rc = ensure_arena(c, off);
-  if (rc) return rc;
-  rc = ensure_hbuf(c, table_bytes + align256(sizeof(double) * n) + align256(sizeof(long) * n) + 512);
-  if (rc) return rc;
-  char* A = c->arena;
+  const size_t host_need = table_bytes + align256(sizeof(double) * n) + align256(sizeof(long) * n) + 512;
+  char* A;
+  char* h;
+  if (fixed) {
+    // caller-provided memory: refuse what does not fit, allocate nothing
+    if (off > fixed->dev_bytes) {
+      char msg[160];
+      std::snprintf(msg, sizeof(msg),
+                    "the update needs %zu bytes of device scratch; the fixed scratch holds %zu",
+                    off, fixed->dev_bytes);
+      return fail(c, MFM_ERR_NOMEM, msg);
+    }
+    if (host_need > fixed->host_bytes)
+      return fail(c, MFM_ERR_NOMEM, "the update's tables exceed the fixed host staging");
+    A = fixed->dev;
+    h = fixed->host;
+  } else {
+    int rc = ensure_arena(c, off);
+    if (rc) return rc;
+    rc = ensure_hbuf(c, host_need);
+    if (rc) return rc;
+    A = c->arena;
+    h = c->hbuf;
+  }
 
   // host staging of the tables
-  char* h = c->hbuf;
   DevPatch* hp = reinterpret_cast<DevPatch*>(h);
   for (int q = 0; q < n; ++q) {
     const PatchIn& P = in[q];
@@ -521,7 +540,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     if (e2 != hipSuccess) return hipfail(c, e2, "dt reduction launch");
     return MFM_OK;
   };
-  if (c->graphs && !c->timer) {
+  if (c->graphs && !c->timer && !fixed) {
     // The kernels read every per-call value (dt, time, constants, tables)
     // from the arena, refreshed by the copy above; the sequence itself only
     // depends on the arena layout and the list sizes.
@@ -559,8 +578,8 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     int rc2 = enqueue(s);
     if (rc2 != MFM_OK) return rc2;
   }
-  double* hdt = reinterpret_cast<double*>(c->hbuf + table_bytes);
-  long* hid = reinterpret_cast<long*>(c->hbuf + table_bytes + align256(sizeof(double) * n));
+  double* hdt = reinterpret_cast<double*>(h + table_bytes);
+  long* hid = reinterpret_cast<long*>(h + table_bytes + align256(sizeof(double) * n));
   e = hipMemcpyAsync(hdt, A + off_dto, sizeof(double) * n, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipMemcpyAsync(hid, A + off_ido, sizeof(long) * n, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -842,7 +861,14 @@ int mfm_ctx_create(const mfm_params* p, mfm_ctx** out) {
   mfm_ctx* c = new mfm_ctx();
   c->p = *p;
   c->C = C;
+  c->err.reserve(1024);  // error messages then need no allocation
   (void)hipGetDevice(&c->device);
+  rc = mfm::host::fixed_scratch_setup(c);
+  if (rc != MFM_OK) {
+    std::fprintf(stderr, "mfm_ctx_create: %s\n", c->err.c_str());
+    delete c;
+    return rc;
+  }
   *out = c;
   return MFM_OK;
 }

@@ -35,6 +35,10 @@ struct mfm_ctx {
   // work walk of the region kernels (row tile in (y, z), XCD renumbering)
   int walk_tile = 8;
   int walk_xcd = 1;
+  // host staging of the allocation-free entry point (tables and results of
+  // one patch); part of the context, so that entry point needs no allocation
+  static constexpr size_t kFixedHostBytes = 64 * 1024;
+  alignas(256) char fixed_host[kFixedHostBytes];
 };
 
 namespace mfm {
@@ -70,6 +74,43 @@ int fail(mfm_ctx* c, int code, const std::string& m);
 int hipfail(mfm_ctx* c, hipError_t e, const char* what);
 int build_consts(const mfm_params* p, Consts& C, std::string& why);
 
+// Caller-provided memory for run_batch instead of the context's growing
+// arena and staging (the allocation-free entry point): dev holds the patch
+// scratch, tables and reductions; host the staged tables and results.
+struct FixedScratch {
+  char* dev;
+  size_t dev_bytes;
+  char* host;
+  size_t host_bytes;
+};
+
+// A vector with N elements of inline storage: lists of one patch's update
+// fit inline, so building them touches no heap.
+template <typename T, int N>
+struct SmallVec {
+  T inl[N];
+  std::vector<T> ext;
+  int n = 0;
+  void push_back(const T& x) {
+    if (n < N) {
+      inl[n] = x;
+    } else {
+      if (n == N) ext.assign(inl, inl + N);
+      ext.push_back(x);
+    }
+    ++n;
+  }
+  void resize(int m) {
+    if (m > N) ext.resize(m);
+    n = m;
+  }
+  int size() const { return n; }
+  T* data() { return n > N ? ext.data() : inl; }
+  const T* data() const { return n > N ? ext.data() : inl; }
+  T& operator[](int i) { return data()[i]; }
+  const T& operator[](int i) const { return data()[i]; }
+};
+
 // One patch of a batched update.
 struct PatchIn {
   mfm_fab U;
@@ -82,7 +123,11 @@ struct PatchIn {
 
 int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
               double* new_dt, int* min_cell, double* level_dt, int* min_patch,
-              hipStream_t s);
+              hipStream_t s, const FixedScratch* fixed = nullptr);
+// Resolves, once per device, the address of the fixed scratch of the
+// allocation-free entry point on the context's device (mfm_noalloc.hip).
+int fixed_scratch_setup(mfm_ctx* c);
+
 // computeDt of one box with the constants Cq.
 int run_dt(mfm_ctx* c, const Consts& Cq, const mfm_fab& U, const mfm_box& box,
            double dx, double* dt, int* min_cell, hipStream_t s);

@@ -91,6 +91,7 @@ EXPORTS = [
     "mfm_params_set_orszag", "mfm_params_set_helio", "mfm_num_components",
     "mfm_num_ghost_cells", "mfm_ctx_create", "mfm_ctx_destroy",
     "mfm_ctx_set_dedner", "mfm_ctx_last_error", "mfm_patch_update_dev",
+    "mfm_patch_update_noalloc_dev",
     "mfm_patch_update_host", "mfm_patch_compute_dt_dev",
     "mfm_level_update_dev", "mfm_init_problem_dev", "mfm_fill_periodic_dev",
     "mfm_dev_count", "mfm_dev_alloc", "mfm_dev_free", "mfm_dev_memcpy",
@@ -168,6 +169,9 @@ def load(path=LIB_PATH):
                                        C.POINTER(C.c_double)]
     lib.mfm_ctx_last_error.restype = C.c_char_p
     lib.mfm_patch_update_dev.argtypes = [
+        C.c_void_p, Fab, Box, C.c_double, C.c_double, C.c_double,
+        C.POINTER(Fab), C.POINTER(Fab), dp, ip, C.c_void_p]
+    lib.mfm_patch_update_noalloc_dev.argtypes = [
         C.c_void_p, Fab, Box, C.c_double, C.c_double, C.c_double,
         C.POINTER(Fab), C.POINTER(Fab), dp, ip, C.c_void_p]
     lib.mfm_patch_update_host.argtypes = [
@@ -347,8 +351,11 @@ class Context:
 
     def _check(self, rc):
         if rc != OK:
-            msg = self.lib.mfm_ctx_last_error(self.h)
-            raise MfmError(rc, msg.decode() if msg else "")
+            raise MfmError(rc, self.last_error())
+
+    def last_error(self):
+        msg = self.lib.mfm_ctx_last_error(self.h)
+        return msg.decode() if msg else ""
 
     @property
     def ncomp(self):
@@ -393,8 +400,10 @@ class Context:
         return ms.value, n.value, it.value
 
     def update_state(self, U, valid, dt, dx, time=0.0, F=None, divB=None,
-                     stream=None, check=True):
-        """PatchMHDAM::updateState on a device FAB; returns (new_dt, cell)."""
+                     stream=None, check=True, noalloc=False):
+        """PatchMHDAM::updateState on a device FAB; returns (new_dt, cell).
+        noalloc: go through mfm_patch_update_noalloc_dev (scratch in the
+        library's fixed per-device array, no allocation in the call)."""
         nd = C.c_double()
         cell = (C.c_int * 3)()
         Fp = None
@@ -403,9 +412,9 @@ class Context:
             Fp = C.cast(arr, C.POINTER(Fab))
             self._Fkeep = arr
         dp = C.pointer(divB) if divB is not None else None
-        rc = self.lib.mfm_patch_update_dev(
-            self.h, U, valid, dt, dx, time, Fp, dp, C.byref(nd), cell,
-            C.c_void_p(stream) if stream else None)
+        fn = self.lib.mfm_patch_update_noalloc_dev if noalloc else self.lib.mfm_patch_update_dev
+        rc = fn(self.h, U, valid, dt, dx, time, Fp, dp, C.byref(nd), cell,
+                C.c_void_p(stream) if stream else None)
         if check:
             self._check(rc)
             return nd.value, tuple(cell)

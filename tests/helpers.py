@@ -138,8 +138,9 @@ class GpuPatch:
 
 
 def gpu_update(ctx, U, lo, vlo, vhi, dt, dx, time=0.0, want_F=False,
-               want_divB=False):
-    """Run mfm_patch_update_dev on a copy of U; returns a dict like
+               want_divB=False, noalloc=False):
+    """Run mfm_patch_update_dev (or, with noalloc, the allocation-free
+    mfm_patch_update_noalloc_dev) on a copy of U; returns a dict like
     oracle.patch_update."""
     g = GpuPatch(U, lo)
     nc = U.shape[0]
@@ -162,7 +163,7 @@ def gpu_update(ctx, U, lo, vlo, vhi, dt, dx, time=0.0, want_F=False,
                                   vhi[0] - vlo[0] + 1), zero=True)
         dfab = mfmhd.fab(dt_t.ptr, vlo, vhi, 1)
     rc, nd, cell = ctx.update_state(g.fab, mfmhd.Box.of(vlo, vhi), dt, dx, time,
-                                    F=Ffabs, divB=dfab, check=False)
+                                    F=Ffabs, divB=dfab, check=False, noalloc=noalloc)
     mfmhd.synchronize()
     return {"rc": rc, "U": g.numpy(), "dt": nd, "cell": cell,
             "F": [t.numpy() for t in Ft[:dims]] if want_F else None,
