@@ -1,0 +1,426 @@
+"""Pins of the oracle against closed forms and invariants taken from the
+mathematics and physics, not from the reference's code: the single-wave
+structure of the Roe solvers (eigenpairs of the ideal-MHD Jacobian, written
+here from the equations, and Harten's entropy fix), the textbook limiter
+functions, van Leer's flux-vector splitting with Haenel's total-enthalpy
+energy flux, and the kinetic-theory limits and Galilean / rotational
+covariance of the Pauls-Zank charge-exchange rates.  Each of these fails
+under a plausible one-token mistake in the oracle (a swapped limiter, an
+eigenvalue paired with the wrong eigenvector, 8/pi for 4/pi, a dropped term,
+a sign)."""
+import numpy as np
+import pytest
+
+import mfmhd
+
+PI4 = 4.0 * np.pi
+
+
+# ------------------------------------------------------------------ Roe ---
+def prim_jacobian(w, gamma, eight_wave=True):
+    """dW/dt + A dW/dx = 0 of ideal MHD in Gaussian units, primitive
+    W = (rho, u, v, w, p, Bx, By, Bz), x direction.  eight_wave: Powell's
+    form (Bx advected, no Bx' terms in the momentum and pressure rows); else
+    the 7 x 7 system without the Bx row and column (Bx a parameter)."""
+    rho, u, v, ww, p, bx, by, bz = w
+    A = np.zeros((8, 8))
+    A[0, 0], A[0, 1] = u, rho
+    A[1, 1], A[1, 4], A[1, 6], A[1, 7] = u, 1 / rho, by / (PI4 * rho), bz / (PI4 * rho)
+    A[2, 2], A[2, 6] = u, -bx / (PI4 * rho)
+    A[3, 3], A[3, 7] = u, -bx / (PI4 * rho)
+    A[4, 1], A[4, 4] = gamma * p, u
+    A[5, 5] = u
+    A[6, 1], A[6, 2], A[6, 6] = by, -bx, u
+    A[7, 1], A[7, 3], A[7, 7] = bz, -bx, u
+    if not eight_wave:
+        keep = [0, 1, 2, 3, 4, 6, 7]
+        A = A[np.ix_(keep, keep)]
+    return A
+
+
+def cons_jacobian(w, gamma):
+    """dU/dW, U = (rho, rho u, rho v, rho w, E, B)."""
+    rho, u, v, ww, p, bx, by, bz = w
+    J = np.eye(8)
+    J[1, 0], J[1, 1] = u, rho
+    J[2, 0], J[2, 2] = v, rho
+    J[3, 0], J[3, 3] = ww, rho
+    J[4, :] = [0.5 * (u * u + v * v + ww * ww), rho * u, rho * v, rho * ww,
+               1 / (gamma - 1), bx / PI4, by / PI4, bz / PI4]
+    return J
+
+
+def mhd_flux(w, gamma):
+    """Ideal-MHD flux, x direction, Gaussian units; the normal field has no
+    flux (8-wave form)."""
+    rho, u, v, ww, p, bx, by, bz = w
+    b2 = bx * bx + by * by + bz * bz
+    E = p / (gamma - 1) + 0.5 * rho * (u * u + v * v + ww * ww) + b2 / (2 * PI4)
+    pt = p + b2 / (2 * PI4)
+    ub = u * bx + v * by + ww * bz
+    return np.array([rho * u, rho * u * u + pt - bx * bx / PI4, rho * u * v - bx * by / PI4,
+                     rho * u * ww - bx * bz / PI4, u * (E + pt) - bx * ub / PI4, 0.0,
+                     u * by - v * bx, u * bz - ww * bx])
+
+
+def harten(lam, delta):
+    """Harten's entropy fix of |lambda| with width delta."""
+    z = abs(lam)
+    return z if z >= delta else 0.5 * (z * z + delta * delta) / delta
+
+
+STATES = [  # (rho, u, v, w, p, Bx, By, Bz): super- and sub-fast, both signs of Bx
+    np.array([1.3, 0.4, -0.2, 0.15, 0.9, 1.1, 0.8, -0.5]),
+    np.array([0.7, -0.9, 0.3, 0.0, 1.6, -2.2, 0.4, 1.3]),
+    np.array([2.1, 2.5, 0.1, -0.3, 0.4, 0.6, -1.7, 0.2]),
+]
+
+
+def single_wave_check(oracle_mod, W0, gamma, del2, eight_wave):
+    p = mfmhd.orszag_params(gamma)
+    p.entropy_fix = del2
+    A = prim_jacobian(W0, gamma, eight_wave)
+    lam, R = np.linalg.eig(A)
+    assert np.allclose(lam.imag, 0) and np.allclose(R.imag, 0)
+    lam, R = lam.real, R.real
+    # the fast speed of the state, from the spectrum: delta of the fix
+    cf = max(lam) - W0[1]
+    delta = del2 * (abs(W0[1]) + abs(W0[2]) + abs(W0[3]) + cf)
+    J = cons_jacobian(W0, gamma)
+    eps = 1e-6 * max(1.0, np.max(np.abs(W0)))
+    nfixed = 0
+    for k in range(len(lam)):
+        r = R[:, k]
+        if not eight_wave:
+            r = np.insert(r, 5, 0.0)  # equal normal field on both sides
+        wl, wr = W0 - 0.5 * eps * r, W0 + 0.5 * eps * r
+        if eight_wave:
+            F = oracle_mod.flux_roe8(p, wl, wr, 0)
+            comps = list(range(8))
+        else:
+            F = oracle_mod.flux_dedner(p, 1.0, wl, wr, 0)[:8]
+            comps = [0, 1, 2, 3, 4, 6, 7]  # Bn comes from the GLM block
+        D = F - 0.5 * (mhd_flux(wl, gamma) + mhd_flux(wr, gamma))
+        a = harten(lam[k], delta)
+        nfixed += abs(lam[k]) < delta
+        want = -0.5 * eps * a * (J @ r)
+        scale = np.max(np.abs(want)) + 1e-300
+        assert np.max(np.abs(D[comps] - want[comps])) <= 1e-4 * scale, (k, lam[k], D, want)
+    return nfixed
+
+
+@pytest.mark.parametrize("eight_wave", [True, False], ids=["RIEMANNF", "RIEMANN7F"])
+@pytest.mark.parametrize("W0", STATES)
+def test_roe_single_wave_jumps(oracle_mod, W0, eight_wave):
+    """A jump along one right eigenvector r_k of the averaged state is
+    dissipated as -1/2 |lambda_k| (dU/dW) r_k: pins every eigenvalue, its
+    pairing with its eigenvector, and the characteristic projections.
+    del2 small: no wave inside the entropy fix."""
+    nfixed = single_wave_check(oracle_mod, W0, 5.0 / 3.0, 1e-3, eight_wave)
+    assert nfixed == 0
+
+
+@pytest.mark.parametrize("eight_wave", [True, False], ids=["RIEMANNF", "RIEMANN7F"])
+def test_roe_entropy_fix_is_hartens(oracle_mod, eight_wave):
+    """With a wide fix (del2 = 0.6) the slow, Alfven and entropy waves of
+    a slowly moving state fall inside it and are dissipated with Harten's
+    (lambda^2 + delta^2) / (2 delta); the fast waves keep |lambda|."""
+    W0 = np.array([1.0, 0.05, 0.02, -0.01, 1.0, 0.9, 0.7, 0.4])
+    nfixed = single_wave_check(oracle_mod, W0, 5.0 / 3.0, 0.6, eight_wave)
+    assert nfixed >= 4
+
+
+# ------------------------------------------------------------- limiters ---
+def mm(a, b):
+    return 0.5 * (np.sign(a) + np.sign(b)) * min(abs(a), abs(b))
+
+
+def tvd_phi(name, r):
+    """Sweby-form limiter functions phi(r), r = D-/D+ (LeVeque, Finite
+    Volume Methods, sec. 6.12): face value w0 +- 1/2 phi(r) D+."""
+    if name == "minmod":
+        return max(0.0, min(1.0, r))
+    if name == "superbee":  # Roe (1985)
+        return max(0.0, min(2 * r, 1.0), min(r, 2.0))
+    if name == "mc":  # van Leer (1977) monotonized central
+        return max(0.0, min(2 * r, 0.5 * (1 + r), 2.0))
+    if name == "harmonic":  # van Leer (1974)
+        return (r + abs(r)) / (1 + abs(r))
+    if name == "vanalbada":  # van Albada, van Leer, Roberts (1982)
+        return (r * r + r) / (r * r + 1) if r > 0 else 0.0
+    raise KeyError(name)
+
+
+def textbook_faces(name, wm, w0, wp):
+    """(W-, W+) of the cell from its neighbours by the published method."""
+    dm, dp = w0 - wm, wp - w0
+    if name in ("minmod", "superbee", "mc", "harmonic", "vanalbada"):
+        s = 0.5 * tvd_phi(name, dm / dp) * dp
+        return w0 - s, w0 + s
+    if name == "koren":
+        # Koren (1988) smooth limiter of the kappa = 1/3 scheme:
+        # w_{i+1/2} = w_i + 1/2 phi(R) D-, R = D+/D-,
+        # phi(R) = (2R^2 + R) / (2R^2 - R + 2); mirrored for w_{i-1/2}
+        # (and phi = 0 at an extremum, R <= 0)
+        phi = lambda R: (2 * R * R + R) / (2 * R * R - R + 2) if R > 0 else 0.0
+        return w0 - 0.5 * phi(dm / dp) * dp, w0 + 0.5 * phi(dp / dm) * dm
+    if name == "kappa13":
+        # limited kappa-scheme, kappa = 1/3, b = (3 - kappa)/(1 - kappa)
+        # (van Leer 1979; Hirsch vol. 2 sec. 21.4):
+        # w_{i+1/2} = w_i + 1/4[(1-k) mm(D-, b D+) + (1+k) mm(D+, b D-)]
+        k = 1.0 / 3.0
+        b = (3 - k) / (1 - k)
+        right = w0 + 0.25 * ((1 - k) * mm(dm, b * dp) + (1 + k) * mm(dp, b * dm))
+        left = w0 - 0.25 * ((1 - k) * mm(dp, b * dm) + (1 + k) * mm(dm, b * dp))
+        return left, right
+    if name in ("weno3", "weno3yc"):
+        # third-order WENO (Jiang & Shu 1996), eps = 1e-6, linear weights
+        # 2/3 (central stencil) and 1/3 (upwind stencil); the Yamaleev-
+        # Carpenter (2009) weights alpha = d (1 + tau / (beta + eps)),
+        # tau = (D+ - D-)^2
+        eps = 1e-6
+
+        def face(d_in, d_out):  # value at the face towards d_out
+            b_c, b_u = d_out * d_out, d_in * d_in
+            if name == "weno3":
+                ac, au = (2 / 3) / (eps + b_c) ** 2, (1 / 3) / (eps + b_u) ** 2
+            else:
+                tau = (d_out - d_in) ** 2
+                ac, au = (2 / 3) * (1 + tau / (b_c + eps)), (1 / 3) * (1 + tau / (b_u + eps))
+            wc = ac / (ac + au)
+            return 0.5 * (wc * d_out + (1 - wc) * d_in)
+        return w0 - face(dp, dm), w0 + face(dm, dp)
+    raise KeyError(name)
+
+
+LIMS = [("minmod", mfmhd.LIM_MINMOD), ("superbee", mfmhd.LIM_SUPERBEE),
+        ("mc", mfmhd.LIM_MC), ("harmonic", mfmhd.LIM_HARMONIC),
+        ("vanalbada", mfmhd.LIM_VANALBADA), ("koren", mfmhd.LIM_KOREN),
+        ("kappa13", mfmhd.LIM_VANLEER), ("weno3", mfmhd.LIM_WENO3),
+        ("weno3yc", mfmhd.LIM_WENO3YC)]
+
+
+@pytest.mark.parametrize("name,lim", LIMS, ids=[n for n, _ in LIMS])
+def test_limiters_textbook(oracle_mod, name, lim):
+    """Every limiter against its published formula on monotone, non-linear
+    triples (slope ratios from 1/20 to 20, where the limiters differ) and,
+    for the smooth ones, on non-monotone triples too."""
+    rng = np.random.default_rng(17)
+    smooth = name in ("koren", "weno3", "weno3yc")  # defined at extrema too
+    n = 0
+    for _ in range(400):
+        w0 = rng.uniform(-2, 2)
+        dp = rng.choice([-1, 1]) * 10 ** rng.uniform(-1.5, 0.5)
+        r = 10 ** rng.uniform(-1.3, 1.3)
+        if name == "vanalbada":
+            # the reference regularises with 1e-6: keep D+ D- >> 1e-6
+            dp = rng.choice([-1, 1]) * rng.uniform(0.5, 3.0)
+            r = 10 ** rng.uniform(-1.0, 1.0)
+        if smooth and rng.uniform() < 0.3:
+            r = -r
+        dm = r * dp
+        wm, wp = w0 - dm, w0 + dp
+        got = oracle_mod.limiter(lim, wm, w0, wp)
+        want = textbook_faces(name, wm, w0, wp)
+        # van Albada and Koren are regularised (1e-6, 1e-10) in the
+        # reference's forms
+        tol = {"vanalbada": 1e-4, "koren": 1e-6}.get(name, 1e-12)
+        for g, w in zip(got, want):
+            assert abs(g - w) <= tol * (abs(dp) + abs(dm)), (name, wm, w0, wp, got, want)
+        n += 1
+    assert n == 400
+
+
+def test_limiters_differ_where_it_matters(oracle_mod):
+    """The textbook forms above really separate the limiters: on a
+    non-linear monotone triple no two of them give the same face value."""
+    vals = [oracle_mod.limiter(lim, 0.0, 1.0, 1.6) for _, lim in LIMS]
+    assert len(set(np.round([v[1] for v in vals], 12))) == len(vals)
+    assert len(set(np.round([v[0] for v in vals], 12))) == len(vals)
+
+
+# ------------------------------------------------------------ van Leer ---
+def test_vanleer_subsonic_split_closed_form(oracle_mod):
+    """Subsonic left state against a supersonic right state leaving to the
+    right (so F- of the right state is 0): the flux is van Leer's split
+    F+(W_L) = rho a (M+1)^2/4 (1, ((gamma-1)u + 2a)/gamma, v, w, H), H the
+    total enthalpy (Haenel's energy flux); and the mirror case for F-."""
+    for gamma in (5.0 / 3.0, 1.4):
+        p = mfmhd.orszag_params(gamma)
+        rng = np.random.default_rng(3)
+        for _ in range(50):
+            rho, pr = rng.uniform(0.3, 3), rng.uniform(0.2, 2)
+            a = np.sqrt(gamma * pr / rho)
+            u = rng.uniform(-0.95, 0.95) * a
+            v, w = rng.uniform(-1, 1, 2)
+            H = (pr / (gamma - 1) + 0.5 * rho * (u * u + v * v + w * w) + pr) / rho
+            m = 0.25 * rho * a * (u / a + 1) ** 2
+            want = m * np.array([1, ((gamma - 1) * u + 2 * a) / gamma, v, w, H])
+            right = np.array([1.0, 5.0, 0.1, 0.2, 0.5])  # u = 5 > a: F- = 0
+            got = oracle_mod.flux_vanleer(p, [rho, u, v, w, pr], right, 0)
+            assert np.allclose(got, want, rtol=1e-13, atol=1e-14)
+            # F-(W_R) with a supersonic left state leaving to the left
+            mneg = -0.25 * rho * a * (u / a - 1) ** 2
+            wneg = mneg * np.array([1, ((gamma - 1) * u - 2 * a) / gamma, v, w, H])
+            left = np.array([1.0, -5.0, 0.1, 0.2, 0.5])
+            got = oracle_mod.flux_vanleer(p, left, [rho, u, v, w, pr], 0)
+            assert np.allclose(got, wneg, rtol=1e-13, atol=1e-14)
+
+
+# ------------------------------------------------------ charge exchange ---
+def ce_params(nf):
+    p = mfmhd.helio_params(lismB=3e-6, lismV=2.6e6, sunN=7.0, sunV=4.5e7,
+                           sunXYZ=(0.5, 0.5, 0.5), R0=0.1, initR=0.3, netN=0.1,
+                           dx=1 / 64, num_cells=(64, 64, 64))
+    p.nfluids = nf
+    return p
+
+
+def ce_state(rho, u, pr, neutrals):
+    w = list([rho] + list(u) + [pr, 0.3, -0.2, 0.1])
+    for (r, un, pn) in neutrals:
+        w += [r] + list(un) + [pn]
+    return np.array(w, dtype=float)
+
+
+def groups(nf):
+    """(rho, momentum slice, energy) component indices per fluid."""
+    g = [(0, slice(1, 4), 4)]
+    for f in range(nf - 1):
+        b = 8 + 5 * f
+        g.append((b, slice(b + 1, b + 4), b + 4))
+    return g
+
+
+@pytest.mark.parametrize("nf", [2, 3, 4])
+@pytest.mark.parametrize("reg", [1, 2, 3])
+def test_charge_exchange_galilean_covariance(oracle_mod, nf, reg):
+    """Shifting every velocity by V leaves the mass sources unchanged and
+    moves momentum and energy sources as S_m' = S_m + V S_rho and
+    S_E' = S_E + V.S_m + V^2/2 S_rho, for each fluid; rotating every
+    velocity rotates the momentum sources.  Any dropped or mis-signed term
+    of the rates breaks this."""
+    p = ce_params(nf)
+    rng = np.random.default_rng(nf * 10 + reg)
+    for _ in range(20):
+        neu = [(rng.uniform(0.05, 0.5), rng.uniform(-1.5, 1.5, 3), rng.uniform(0.01, 0.3))
+               for _ in range(nf - 1)]
+        w = ce_state(rng.uniform(0.1, 2), rng.uniform(-2, 2, 3), rng.uniform(0.01, 1), neu)
+        S = oracle_mod.charge_exchange(p, w, reg, 0.01)
+        V = rng.uniform(-1, 1, 3)
+        ws = w.copy()
+        for (_, m, _) in groups(nf):
+            ws[m] += V
+        Ss = oracle_mod.charge_exchange(p, ws, reg, 0.01)
+        scale = np.max(np.abs(S)) * (1 + np.sum(V * V))
+        for (ir, m, ie) in groups(nf):
+            assert abs(Ss[ir] - S[ir]) <= 1e-11 * scale
+            assert np.all(np.abs(Ss[m] - (S[m] + V * S[ir])) <= 1e-11 * scale)
+            want_e = S[ie] + V @ S[m] + 0.5 * (V @ V) * S[ir]
+            assert abs(Ss[ie] - want_e) <= 1e-11 * scale
+        # rotation about a random axis
+        q = rng.normal(size=3)
+        q /= np.linalg.norm(q)
+        t = rng.uniform(0, 2 * np.pi)
+        K = np.array([[0, -q[2], q[1]], [q[2], 0, -q[0]], [-q[1], q[0], 0]])
+        Rm = np.eye(3) + np.sin(t) * K + (1 - np.cos(t)) * K @ K
+        wr = w.copy()
+        for (_, m, _) in groups(nf):
+            wr[m] = Rm @ w[m]
+        Sr = oracle_mod.charge_exchange(p, wr, reg, 0.01)
+        for (ir, m, ie) in groups(nf):
+            assert abs(Sr[ir] - S[ir]) <= 1e-11 * scale
+            assert abs(Sr[ie] - S[ie]) <= 1e-11 * scale
+            assert np.all(np.abs(Sr[m] - Rm @ S[m]) <= 1e-11 * scale)
+
+
+def test_charge_exchange_cold_beam_limit(oracle_mod):
+    """Cold plasma and neutrals (p, p_H -> 0): every exchange turns a
+    proton of velocity u into one of velocity u_H, so the plasma momentum
+    source is (rate) (u_H - u) and its energy source (rate) (u_H^2 - u^2)/2,
+    with rate = the neutral's mass loss (the region where the neutral
+    population is not born).  The cross section cancels in the ratio."""
+    p = ce_params(2)
+    dt = 0.02
+    for u, uh in (((1.0, 0.2, -0.1), (0.1, -0.3, 0.2)), ((-2.0, 0.5, 0.0), (0.3, 0.3, 0.3))):
+        u, uh = np.array(u), np.array(uh)
+        w = ce_state(0.7, u, 1e-14, [(0.2, uh, 1e-14)])
+        S = oracle_mod.charge_exchange(p, w, 2, dt)  # region 2: neutral 1 only loses
+        rate = -S[8] / dt  # the neutral's mass loss per unit time
+        assert rate > 0
+        assert np.allclose(S[1:4] / dt, rate * (uh - u), rtol=1e-6, atol=0)
+        assert np.isclose(S[4] / dt, 0.5 * rate * (uh @ uh - u @ u), rtol=1e-6, atol=0)
+        assert S[0] == 0.0
+        # and the neutral loses what it carries: momentum rate u_H, energy u_H^2/2
+        assert np.allclose(S[9:12] / dt, -rate * uh, rtol=1e-6, atol=0)
+        assert np.isclose(S[12] / dt, -0.5 * rate * (uh @ uh), rtol=1e-6, atol=0)
+
+
+def test_charge_exchange_mean_relative_speed(oracle_mod):
+    """The exchange rate is sigma(U*) rho rho_H U*, and with no drift U* is
+    the mean relative speed of two Maxwellians, sqrt(4/pi (v_p^2 + v_H^2))
+    with v^2 = 2kT/m (= p/rho for the plasma, 2 p_H / rho_H for the
+    neutrals).  A cold beam at that speed has the same U*, hence the same
+    cross section and the same mass rate."""
+    p = ce_params(2)
+    dt = 0.01
+    for (pp, ph) in ((0.3, 0.05), (0.02, 0.4), (1.0, 1.0)):
+        rho, rh = 0.8, 0.25
+        vp2, vh2 = pp / rho, 2 * ph / rh
+        ustar = np.sqrt(4 / np.pi * (vp2 + vh2))
+        warm = ce_state(rho, (0.3, 0.1, 0.0), pp, [(rh, (0.3, 0.1, 0.0), ph)])
+        cold = ce_state(rho, (0.3 + ustar, 0.1, 0.0), 1e-14, [(rh, (0.3, 0.1, 0.0), 1e-14)])
+        rw = -oracle_mod.charge_exchange(p, warm, 2, dt)[8]
+        rc = -oracle_mod.charge_exchange(p, cold, 2, dt)[8]
+        assert np.isclose(rw, rc, rtol=1e-6, atol=0), (pp, ph, rw, rc)
+
+
+def test_charge_exchange_detailed_balance(oracle_mod):
+    """No drift: at equal temperatures plasma and neutrals exchange no
+    energy (thermal equilibrium); a hotter plasma loses energy to the
+    neutrals, a colder one gains."""
+    p = ce_params(2)
+    rho, rh = 0.6, 0.3
+    for T in (0.1, 1.0):
+        pp, ph = rho * T, 0.5 * rh * T  # v_p^2 = p/rho = T = v_H^2 = 2 p_H/rho_H
+        w = ce_state(rho, (0.2, -0.1, 0.3), pp, [(rh, (0.2, -0.1, 0.3), ph)])
+        S = oracle_mod.charge_exchange(p, w, 2, 0.01)
+        rate = -S[8]
+        u = np.array([0.2, -0.1, 0.3])
+        # no momentum and no energy moves between the species
+        assert np.all(np.abs(S[1:4]) <= 1e-12 * rate * (1 + u @ u))
+        assert abs(S[4]) <= 1e-12 * rate * (1 + u @ u)
+    hot = ce_state(rho, (0, 0, 0), rho * 1.0, [(rh, (0, 0, 0), 0.5 * rh * 0.2)])
+    cold = ce_state(rho, (0, 0, 0), rho * 0.2, [(rh, (0, 0, 0), 0.5 * rh * 1.0)])
+    assert oracle_mod.charge_exchange(p, hot, 2, 0.01)[4] < 0
+    assert oracle_mod.charge_exchange(p, cold, 2, 0.01)[4] > 0
+
+
+@pytest.mark.parametrize("cold", ["neutral", "plasma"])
+def test_charge_exchange_one_cold_species(oracle_mod, cold):
+    """Exact kinetic-theory rates that Pauls' formulas reproduce when one
+    species is cold (thermal speed a of the warm one, v^2 = 2kT/m) and the
+    drift du is small: per exchange the plasma loses the relative-speed-
+    weighted energy a^2 of a warm proton (cold neutrals) or gains a^2 from a
+    warm atom (cold plasma), and the momentum exchange is 4/3 R du, R the
+    mass exchange rate (E|g|g = 4/3 E|g| du for a slightly drifting
+    Maxwellian).  Pins the thermal coefficients of U*_M, U*_E and their
+    pairing with the plasma or the neutral temperature."""
+    p = ce_params(2)
+    dt = 0.01
+    rho, rh = 0.9, 0.3
+    a2 = 0.7
+    du = 1e-5 * np.sqrt(a2)
+    if cold == "neutral":
+        pp, ph = rho * a2, 1e-16
+    else:
+        pp, ph = 1e-16, 0.5 * rh * a2
+    w = ce_state(rho, (du, 0.0, 0.0), pp, [(rh, (0.0, 0.0, 0.0), ph)])
+    S = oracle_mod.charge_exchange(p, w, 2, dt)
+    R = -S[8] / dt  # neutral mass loss rate (not born in region 2)
+    assert R > 0
+    if cold == "neutral":
+        assert np.isclose(S[4] / dt, -R * a2, rtol=1e-7, atol=0)
+    else:
+        assert np.isclose(S[4] / dt, R * a2, rtol=1e-7, atol=0)
+    assert np.isclose(S[1] / dt, -4.0 / 3.0 * R * du, rtol=1e-6, atol=0)
