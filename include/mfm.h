@@ -63,7 +63,9 @@ enum { MFM_COORD_CARTESIAN = 0, MFM_COORD_AXISYMMETRIC = 1 };
 /* physical problem supplying boundary conditions and problem sources     */
 enum { MFM_PROBLEM_NONE = 0,      /* all ghosts supplied by the caller    */
        MFM_PROBLEM_RIEMANN = 1,   /* problems/RiemannProblem.cpp         */
-       MFM_PROBLEM_HELIO = 2 };   /* swlism/HeliosphericProblem.cpp      */
+       MFM_PROBLEM_HELIO = 2,     /* swlism/HeliosphericProblem.cpp      */
+       MFM_PROBLEM_SWLISM = 3 };  /* swlism/SWLISMProblem.cpp: 2-D,      */
+                                  /* coord_sys = MFM_COORD_AXISYMMETRIC   */
 /* arithmetic of reconstruction + Riemann fluxes                          */
 enum { MFM_PREC_FP64 = 0,         /* everything FP64 (parity build)       */
        MFM_PREC_FP32_FLUX = 1 };  /* FP32 faces/fluxes, FP64 state, FP64 */
@@ -126,10 +128,13 @@ typedef struct mfm_params {
   double lism_bx, lism_by, lism_bz;
   double net_rho, net_p;
   double tmlim, reg_safe_z_top, reg_safe_z_bot;
-  double r0_dt_ignore;            /* reserved (unused): the dt exclusion    */
-                                  /* radius is R0 - 2 dx of the dx passed   */
-                                  /* with each call, as m_R0dtIgnore in     */
-                                  /* HeliosphericProblem::define (:497)     */
+  double r0_dt_ignore;            /* SW/LISM: cells within this distance  */
+                                  /* of the sun do not limit dt            */
+                                  /* (m_R0dtIgnore = R0 - dx of the        */
+                                  /* coarsest level, SWLISMProblem.cpp:153);*/
+                                  /* heliospheric: unused, the radius is   */
+                                  /* R0 - 2 dx of the dx of each call      */
+                                  /* (HeliosphericProblem::define :497)    */
   /* /chargeexcm/ (SETCHARGEEX_PARS, swlism/ChargeExchange2F.ChF:4-21)     */
   double ce_scale_vel;            /* lismV (cm/s)                         */
   double ce_sigma_lindsay;        /* 1e-16 * scaleLen * lismN             */
@@ -167,6 +172,17 @@ int mfm_params_set_helio(mfm_params* p, double gamma,
                          double regSafeDZ, double dx_level0,
                          const int num_cells[3]);
 
+/* SETSWLISM + SETCHARGEEX_PARS (swlism/SWLISMF.ChF:6-104,
+ * swlism/SWLISMProblem.cpp:68-256): the 2-D axisymmetric solar-wind / LISM
+ * problem (coord_sys = 1, the symmetry axis at y = 0 is the low y boundary),
+ * num_cells[0] x num_cells[1] cells of dx_level0 (= domain_length /
+ * num_cells[0]); lengths in AU, the sun node-snapped, flow along +x. */
+int mfm_params_set_swlism(mfm_params* p, double gamma, double lismN, double lismV,
+                          double lismT, double lismB, const double sunXY[2],
+                          double R0, double sunN, double sunV, double sunT,
+                          double initR, double netN, double TMLIM,
+                          double dx_level0, const int num_cells[2]);
+
 /* Number of state components for these parameters (EqSysMHDMF layout). */
 int mfm_num_components(const mfm_params* p);
 /* PatchMHDAM::numGhostCells: 4 (RK2), 3 (Hancock), 2 (first order). */
@@ -197,7 +213,8 @@ enum {
   MFM_KT_FLUX_NEUTRAL = 2, /* interior faces, neutral van Leer fluxes    */
   MFM_KT_FLUX_BC = 3,      /* faces on a physical boundary (fluxBC)      */
   MFM_KT_UPDATE = 4,       /* flux divergence, sources, floors, CFL      */
-  MFM_KT_PREDICT = 5       /* Hancock predictor (cell-local half step)   */
+  MFM_KT_PREDICT = 5,      /* Hancock predictor (cell-local half step)   */
+  MFM_KT_STAGE = 6         /* fused stage: interior faces + update       */
 };
 int mfm_ctx_set_timing(mfm_ctx* ctx, int on);
 /* hipGraph replay (no counterpart in the reference): when on, the launch
@@ -220,6 +237,16 @@ int mfm_ctx_autotune(mfm_ctx* ctx, mfm_fab U, mfm_box valid, double dt,
                      int* xcd_order, double* best_ms);
 int mfm_ctx_get_timing(mfm_ctx* ctx, int stage, int kind, double* ms,
                        long* launches, double* items);
+/* Kernel structure of RK2 and first-order stages (no counterpart in the
+ * reference).  fused = 0 (default): the separate per-direction flux
+ * kernels and k_update.  fused = 1: one launch per stage solves the
+ * interior face fluxes of a 16 x 16 column tile into LDS and updates its
+ * cells, walking z-chunks of tz planes (default 16), after k_prim and the
+ * boundary-face kernel; the face fluxes never reach HBM, but the kernel
+ * needs more registers than the separate ones and runs at lower occupancy
+ * (slower on the 256^3 workload).  Both give the same results; the Hancock
+ * scheme and characteristic slopes always use the separate kernels.       */
+int mfm_ctx_set_stage_kernel(mfm_ctx* ctx, int fused, int tz);
 
 /* ---- one patch: PatchMHDAM::updateState ---------------------------------
  * U      : device FAB, box = valid grown by >= numGhostCells, ghosts filled
@@ -405,8 +432,8 @@ int mfm_fill_time_coef(double time, double t_coarse_new, double dt_coarse,
 /* ---- ensemble serving ----------------------------------------------------
  * No counterpart in the reference, which runs one simulation per process.
  * A server owns independent solves of one equation class (spacedim,
- * nfluids, dedner, Riemann solver family (Roe or Rusanov), time scheme,
- * precision and characteristic slopes of `cls`).  Each request brings
+ * coordinate system, nfluids, dedner, Riemann solver family (Roe or
+ * Rusanov), time scheme, precision and characteristic slopes of `cls`).  Each request brings
  * its own parameters (problem constants and domain = its grid), dx, step
  * count and CFL number.  mfm_server_step admits pending requests in arrival
  * order while their device bytes (state + update scratch) fit the HBM

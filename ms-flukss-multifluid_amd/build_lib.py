@@ -4,7 +4,9 @@ Each translation unit is compiled with its own floating-point rules:
   mfm_kernels_fp64.hip, mfm_init.hip, mfm_fill.hip, mfm_sync.hip :
                                        -ffp-contract=off
                                        (reference rounding)
-  mfm_kernels_fp32.hip               : FMA contraction, fast FP32 div/sqrt,
+  mfm_stage_fp64.hip                 : -ffp-contract=off
+  mfm_kernels_fp32.hip, mfm_stage_fp32.hip
+                                     : FMA contraction, fast FP32 div/sqrt,
                                        FP32 denormals flushed (a division
                                        is then one reciprocal and a multiply)
 All HIP code is built with packed-FP32 instructions disabled.
@@ -29,6 +31,10 @@ UNITS = [
     ("mfm_kernels_fp32.hip", ["-ffp-contract=fast",
                               "-fno-hip-fp32-correctly-rounded-divide-sqrt",
                               "-fgpu-flush-denormals-to-zero"]),
+    ("mfm_stage_fp64.hip", ["-ffp-contract=off"]),
+    ("mfm_stage_fp32.hip", ["-ffp-contract=fast",
+                            "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+                            "-fgpu-flush-denormals-to-zero"]),
     ("mfm_init.hip", ["-ffp-contract=off"]),
     ("mfm_capi.cpp", ["-ffp-contract=off"]),
     ("mfm_serve.cpp", ["-ffp-contract=off"]),

@@ -98,6 +98,19 @@ struct RegionBuilder {
     nblk += (int)((cells + kBlk - 1) / kBlk);
     v.push_back(r);
   }
+  // a region of the fused stage kernel: one block per 16 x 16 column tile
+  // and z-chunk of tz planes
+  void add_tiles(int patch, const HBox& b, int tz) {
+    long cells = b.size();
+    if (cells <= 0) return;
+    items += cells;
+    Region r;
+    r.patch = patch;
+    for (int d = 0; d < 3; ++d) { r.lo[d] = b.lo[d]; r.n[d] = (int)b.n(d); }
+    r.blk0 = nblk;
+    nblk += (int)(((b.n(0) + 15) / 16) * ((b.n(1) + 15) / 16) * ((b.n(2) + tz - 1) / tz));
+    v.push_back(r);
+  }
 };
 
 int build_consts(const mfm_params* p, Consts& C, std::string& why) {
@@ -121,7 +134,18 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
     why = "GLM cleaning is not run with the Hancock scheme";
     return MFM_ERR_UNSUPPORTED;
   }
-  if (p->coord_sys != MFM_COORD_CARTESIAN) { why = "only Cartesian coordinates"; return MFM_ERR_UNSUPPORTED; }
+  if (p->coord_sys == MFM_COORD_AXISYMMETRIC) {
+    // CS_Axisymmetric: Cartesian fluxes and volumes plus the geometric
+    // sources; with the SW/LISM problem or caller-filled ghosts, in 2-D
+    if (!(p->problem == MFM_PROBLEM_SWLISM || p->problem == MFM_PROBLEM_NONE) ||
+        p->spacedim != 2 || p->dedner || p->time_approx == MFM_TIME_HANCOCK) {
+      why = "coord_sys = 1 runs in 2-D (SW/LISM or caller-filled ghosts), RK2 or first order, without GLM";
+      return MFM_ERR_UNSUPPORTED;
+    }
+  } else if (p->coord_sys != MFM_COORD_CARTESIAN) {
+    why = "coord_sys must be 0 (Cartesian) or 1 (axisymmetric)";
+    return MFM_ERR_UNSUPPORTED;
+  }
   if (p->check_tvd < 0 || p->check_tvd > 2) { why = "check_tvd must be 0, 1 or 2"; return MFM_ERR_ARG; }
   if (p->char_reconstruction && p->precision != MFM_PREC_FP64) {
     why = "char_reconstruction runs in the FP64 build only";
@@ -152,6 +176,16 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
       if (!p->periodic[d]) { why = "Riemann problem: transverse directions must be periodic"; return MFM_ERR_UNSUPPORTED; }
   } else if (p->problem == MFM_PROBLEM_HELIO) {
     if (p->spacedim != 3) { why = "heliospheric problem is 3-D"; return MFM_ERR_UNSUPPORTED; }
+  } else if (p->problem == MFM_PROBLEM_SWLISM) {
+    if (p->spacedim != 2 || p->coord_sys != MFM_COORD_AXISYMMETRIC) {
+      why = "SW/LISM problem is 2-D axisymmetric (coord_sys = 1)";
+      return MFM_ERR_UNSUPPORTED;
+    }
+    if (p->dedner || p->time_approx == MFM_TIME_HANCOCK) {
+      why = "SW/LISM problem runs RK2 or first order without GLM cleaning here";
+      return MFM_ERR_UNSUPPORTED;
+    }
+    if (p->periodic[0] || p->periodic[1]) { why = "SW/LISM boundaries are not periodic"; return MFM_ERR_ARG; }
   } else if (p->problem != MFM_PROBLEM_NONE) {
     why = "unknown problem"; return MFM_ERR_ARG;
   }
@@ -206,7 +240,14 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
   // Fortran default-real literals of the cross section (see include/mfm.h)
   C.c13493 = p->ce_single_literals ? (double)13.493f : 13.493;
   C.c0531 = p->ce_single_literals ? (double)0.531f : 0.531;
-  C.ce_on = (p->problem == MFM_PROBLEM_HELIO && p->nfluids >= 2) ? 1 : 0;
+  C.ce_on = ((p->problem == MFM_PROBLEM_HELIO || p->problem == MFM_PROBLEM_SWLISM) &&
+             p->nfluids >= 2) ? 1 : 0;
+  // CHARGE_EXCHANGE_MF (the SW/LISM problem) rather than _2F/_3F/_4F
+  // SWLISMProblem::explicitSource (swlism/SWLISMProblem.cpp:945-1021) calls
+  // CHARGE_EXCHANGE_MF for 2, 3 and 5 fluids and CHARGE_EXCHANGE_4F for 4
+  C.ce_mf = (p->problem == MFM_PROBLEM_SWLISM && p->nfluids != 4) ? 1 : 0;
+  C.axisym = p->coord_sys == MFM_COORD_AXISYMMETRIC ? 1 : 0;
+  C.r0_dt_ignore = p->r0_dt_ignore;
   C.sin_tilt = std::sin(p->sun_tilt);
   C.cos_tilt = std::cos(p->sun_tilt);
   C.sun_beta = p->problem == MFM_PROBLEM_HELIO ? p->omega * kAU / (p->sun_v * p->lism_v_dim) : 0.0;
@@ -232,6 +273,10 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
   const bool rk2 = p.time_approx == MFM_TIME_RK2;
   const bool han = p.time_approx == MFM_TIME_HANCOCK;
   const int ng = rk2 ? 4 : (han ? 3 : 2);  // PatchMHDAM::numGhostCells
+  // one launch per stage for the interior faces and the update (k_stage);
+  // the Hancock scheme and characteristic slopes keep the separate kernels
+  const bool fused = c->fused && !han && !C0.char_rec;
+  const int tz = std::max(1, c->stage_tz);
   if (n <= 0) return fail(c, MFM_ERR_ARG, "empty patch table");
   for (int r = 1; r < nct; ++r)
     if (Ct[r].dims != C0.dims || Ct[r].nf != C0.nf || Ct[r].dedner != C0.dedner)
@@ -268,7 +313,9 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     g.off_up = off; off += align256((size_t)ucells * C0.nc * 8);
     g.off_w = off; off += align256((size_t)ucells * C0.nc * fs);
     g.off_w2 = off;
-    if (han) off += align256((size_t)ucells * C0.nc * fs);
+    // Hancock: W of the predicted state; fused RK2: W of U*, written by
+    // stage 1 while other blocks still read W of U^n
+    if (han || (fused && rk2)) off += align256((size_t)ucells * C0.nc * fs);
     for (int d = 0; d < dims; ++d) {
       g.fb[d] = g.vb1;
       g.fb[d].hi[d] += 1;
@@ -315,7 +362,8 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     if (rk2) {
       st1_prim.add(q, grow(g.vb1, 2, dims));
       for (int d = 0; d < dims; ++d) add_faces(st1_flux[d], st1_fbc[d], q, g.fb[d], d);
-      st1_upd.add(q, g.vb1);
+      if (fused) st1_upd.add_tiles(q, g.vb1, tz);
+      else st1_upd.add(q, g.vb1);
     } else if (han) {
       // predictor: W on its stencil (value box + 2), one kernel on the cells
       st1_prim.add(q, grow(g.vb1, 2, dims));
@@ -365,7 +413,8 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
       f.hi[d] += 1;
       add_faces(st2_flux[d], st2_fbc[d], q, f, d);
     }
-    st2_upd.add(q, g.vb2);
+    if (fused) st2_upd.add_tiles(q, g.vb2, tz);
+    else st2_upd.add(q, g.vb2);
   }
   constexpr int kLists = 16;
   RegionBuilder* lists[kLists] = {&st1_prim, &st1_flux[0], &st1_flux[1], &st1_flux[2],
@@ -373,9 +422,12 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
                                   &st2_prim, &st2_flux[0], &st2_flux[1], &st2_flux[2],
                                   &st2_fbc[0], &st2_fbc[1], &st2_fbc[2], &st2_upd};
   // table layout inside the arena (after the scratch)
-  size_t off_patch = off; off += align256(sizeof(DevPatch) * n);
+  // fused RK2: a second patch table for stage 2, whose W is the W of U*
+  // that stage 1 wrote (the kernels always read W through the table)
+  const int ntab = (fused && rk2) ? 2 : 1;
+  size_t off_patch = off; off += ntab * align256(sizeof(DevPatch) * n);
   size_t off_reg[kLists];
-  size_t table_bytes = align256(sizeof(DevPatch) * n);
+  size_t table_bytes = ntab * align256(sizeof(DevPatch) * n);
   // constants table after the region lists
 
   for (int l = 0; l < kLists; ++l) {
@@ -436,6 +488,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     D.Up = reinterpret_cast<double*>(A + g.off_up);
     D.W = A + g.off_w;
     D.W2 = han ? A + g.off_w2 : nullptr;
+    D.Wn = (fused && rk2) ? A + g.off_w2 : D.W;
     for (int d = 0; d < 3; ++d) {
       D.ulo[d] = g.ub.lo[d];
       D.un[d] = (int)g.ub.n(d);
@@ -458,9 +511,9 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
       D.divB = P.divB->data;
       for (int e = 0; e < 3; ++e) { D.dlo[e] = db.lo[e]; D.dn[e] = (int)db.n(e); }
     }
-    // HELIOREINIT_DEFAULT's early exit on the FAB bounds
+    // HELIOREINIT_DEFAULT's / SWLISMREINIT's early exit on the FAB bounds
     D.reinit = 0;
-    if (Cq.problem == MFM_PROBLEM_HELIO) {
+    if (Cq.problem == MFM_PROBLEM_HELIO || Cq.problem == MFM_PROBLEM_SWLISM) {
       const double dx = P.dx, sunR = Cq.sun_r0;
       double xl = g.ub.lo[0] * dx - Cq.sun_xc, yl = g.ub.lo[1] * dx - Cq.sun_yc, zl = g.ub.lo[2] * dx - Cq.sun_zc;
       double xh = (g.ub.hi[0] + 1) * dx - Cq.sun_xc, yh = (g.ub.hi[1] + 1) * dx - Cq.sun_yc,
@@ -470,7 +523,15 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
       D.reinit = out ? 0 : 1;
     }
   }
-  size_t hoff = align256(sizeof(DevPatch) * n);
+  if (ntab == 2) {
+    // stage 2 reads (and its ghost slabs write) the W of U*
+    DevPatch* hp2 = reinterpret_cast<DevPatch*>(h + align256(sizeof(DevPatch) * n));
+    for (int q = 0; q < n; ++q) {
+      hp2[q] = hp[q];
+      hp2[q].W = hp[q].Wn;
+    }
+  }
+  size_t hoff = ntab * align256(sizeof(DevPatch) * n);
   for (int l = 0; l < kLists; ++l) {
     std::memcpy(h + hoff, lists[l]->v.data(), sizeof(Region) * lists[l]->v.size());
     hoff += align256(sizeof(Region) * std::max<size_t>(1, lists[l]->v.size()));
@@ -488,6 +549,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     return r;
   };
   const DevPatch* dp = reinterpret_cast<const DevPatch*>(A + off_patch);
+  const DevPatch* dp2 = reinterpret_cast<const DevPatch*>(A + off_patch + (ntab - 1) * align256(sizeof(DevPatch) * n));
   double* dtp = reinterpret_cast<double*>(A + off_dtp);
   long* idp = reinterpret_cast<long*>(A + off_idp);
   const Consts* ct = reinterpret_cast<const Consts*>(A + off_ct);
@@ -500,6 +562,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
       if (d >= dims) { L.flux[d].nblk = 0; L.fbc[d].nblk = 0; }
     }
     L.upd = rl(l0 + 7);
+    L.fused = fused ? 1 : 0;
     return L;
   };
   if (c->timer) c->timer->n = 0;
@@ -507,30 +570,34 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
   // the whole launch sequence of one update (both stages, dt reduction)
   auto enqueue = [&](hipStream_t st) -> int {
     auto launch = [&](const StageLists& L, const StepArgs& S) -> hipError_t {
-      if (p.precision == MFM_PREC_FP32_FLUX)
-        return launch_stage<float>(C0, ct, dp, L, S, dtp, idp, st, c->timer);
-      return launch_stage<double>(C0, ct, dp, L, S, dtp, idp, st, c->timer);
+      const DevPatch* tp = (S.stage == 1 && rk2) ? dp2 : dp;
+      const bool f32 = p.precision == MFM_PREC_FP32_FLUX;
+      hipError_t e3 = f32 ? launch_stage<float>(C0, ct, tp, L, S, dtp, idp, st, c->timer)
+                          : launch_stage<double>(C0, ct, tp, L, S, dtp, idp, st, c->timer);
+      if (e3 != hipSuccess || !L.fused) return e3;
+      return f32 ? launch_stage_fused<float>(C0, ct, tp, L, S, dtp, idp, st, c->timer)
+                 : launch_stage_fused<double>(C0, ct, tp, L, S, dtp, idp, st, c->timer);
     };
     hipError_t e2;
     if (han) {
       // predictorHancock with dt/2, then correctorHancock with dt
-      StepArgs S1{0.5, 0, 0, 0, 0, 0, 0, 0, c->walk_tile, c->walk_xcd, 1};
+      StepArgs S1{0.5, 0, 0, 0, 0, 0, 0, 0, c->walk_tile, c->walk_xcd, 1, tz};
       e2 = launch(stage(0), S1);
       if (e2 != hipSuccess) return hipfail(c, e2, "predictor launch");
       if (c->timer) n_stage1 = c->timer->n;
-      StepArgs S2{1.0, 1, 0, 0, 0, 2, 1, 0, c->walk_tile, c->walk_xcd, 2};
+      StepArgs S2{1.0, 1, 0, 0, 0, 2, 1, 0, c->walk_tile, c->walk_xcd, 2, tz};
       e2 = launch(stage(8), S2);
       if (e2 != hipSuccess) return hipfail(c, e2, "corrector launch");
     } else if (rk2) {
-      StepArgs S1{1.0, 0, 0, 1, 0, 0, 0, 1, c->walk_tile, c->walk_xcd};
+      StepArgs S1{1.0, 0, 0, 1, 0, 0, 0, 1, c->walk_tile, c->walk_xcd, 0, tz};
       e2 = launch(stage(0), S1);
       if (e2 != hipSuccess) return hipfail(c, e2, "stage 1 launch");
       if (c->timer) n_stage1 = c->timer->n;
-      StepArgs S2{0.5, 1, 1, 0, 1, 1, 1, 0, c->walk_tile, c->walk_xcd};
+      StepArgs S2{0.5, 1, 1, 0, 1, 1, 1, 0, c->walk_tile, c->walk_xcd, 0, tz};
       e2 = launch(stage(8), S2);
       if (e2 != hipSuccess) return hipfail(c, e2, "stage 2 launch");
     } else {
-      StepArgs S{1.0, 0, 0, 0, 0, 2, 1, 0, c->walk_tile, c->walk_xcd};
+      StepArgs S{1.0, 0, 0, 0, 0, 2, 1, 0, c->walk_tile, c->walk_xcd, 0, tz};
       e2 = launch(stage(8), S);
       if (e2 != hipSuccess) return hipfail(c, e2, "first-order launch");
     }
@@ -548,7 +615,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
                                   (long long)off_ct, (long long)off_dtp, (long long)off_idp,
                                   (long long)off_dto, (long long)off_ido, dims, p.time_approx,
                                   p.precision, C0.nf, C0.dedner, c->walk_tile,
-                                  c->walk_xcd};
+                                  c->walk_xcd, fused ? 1 : 0, tz};
     for (int l = 0; l < kLists; ++l) {
       key.push_back((long long)off_reg[l]);
       key.push_back((long long)lists[l]->v.size());
@@ -830,6 +897,62 @@ int mfm_params_set_helio(mfm_params* p, double gamma, double lismN, double lismV
   return MFM_OK;
 }
 
+int mfm_params_set_swlism(mfm_params* p, double gamma, double lismN, double lismV,
+                          double lismT, double lismB, const double sunXY[2], double R0,
+                          double sunN, double sunV, double sunT, double initR, double netN,
+                          double TMLIM, double dx_level0, const int num_cells[2]) {
+  if (!p || !sunXY || !num_cells) return MFM_ERR_ARG;
+  if (!(dx_level0 > 0.0) || num_cells[0] < 1 || num_cells[1] < 1) return MFM_ERR_ARG;
+  p->problem = MFM_PROBLEM_SWLISM;
+  p->coord_sys = MFM_COORD_AXISYMMETRIC;
+  p->gamma = gamma;
+  p->spacedim = 2;
+  for (int d = 0; d < 3; ++d) {
+    p->domain_lo[d] = 0;
+    p->domain_hi[d] = d < 2 ? num_cells[d] - 1 : 0;
+    p->periodic[d] = 0;
+  }
+  // the sun on a grid node (SWLISMProblem::input, :146-147); z unused in 2-D
+  p->sun_xc = ((int)std::floor(sunXY[0] / dx_level0 + 0.5)) * dx_level0;
+  p->sun_yc = ((int)std::floor(sunXY[1] / dx_level0 + 0.5)) * dx_level0;
+  p->sun_zc = 0.0;
+  p->sun_r0 = R0;
+  p->r0_dt_ignore = R0 - dx_level0;  // m_R0dtIgnore (:149)
+  p->init_r = initR;
+  p->tmlim = TMLIM;
+  p->lism_t = lismT;
+  p->lism_v_dim = lismV;
+  // SETSWLISM
+  const double rho = lismN * eos_mp;
+  const double pref = rho * lismV * lismV;
+  p->lism_p = (2.0 * eos_k * lismN * lismT) / pref;
+  p->lism_m = 1.0 / std::sqrt(gamma * p->lism_p);
+  p->lism_bx = lismB / std::sqrt(pref);  // lismBX: the field along the flow
+  p->lism_by = 0.0;
+  p->lism_bz = 0.0;
+  p->lism_ux = 1.0;
+  p->lism_uy = 0.0;
+  p->lism_uz = 0.0;
+  p->sun_rho = sunN / lismN;
+  p->sun_p = (2.0 * eos_k * sunN * sunT) / pref;
+  p->sun_v = sunV / lismV;
+  if (netN > 0.0) {
+    p->net_rho = netN / lismN;
+    p->net_p = (eos_k * netN * lismT) / pref;
+  }
+  const double r0 = 1.0;
+  const double p0 = 1.0 / (gamma * p->lism_m * p->lism_m);
+  const double umag = std::sqrt(gamma * p0 / r0);
+  p->smallr = 1.0e-8 * r0;
+  p->smallp = 1.0e-8 * r0 * 0.5 * umag * umag;
+  if (p->smallr < 1.0e-12) p->smallr = 1.0e-12;
+  if (p->smallp < 1.0e-12) p->smallp = 1.0e-12;
+  // SETCHARGEEX_PARS(lismV, scaleLen = 1.5e13, lismN) (:240-247)
+  p->ce_scale_vel = lismV;
+  p->ce_sigma_lindsay = 1.0e-16 * 1.5e13 * lismN;
+  return MFM_OK;
+}
+
 int mfm_num_components(const mfm_params* p) {
   if (!p) return MFM_ERR_ARG;
   return 8 + (p->dedner ? 1 : 0) + 5 * (p->nfluids - 1);
@@ -905,6 +1028,13 @@ int mfm_ctx_set_walk(mfm_ctx* c, int row_tile, int xcd_order) {
   if (!c || row_tile < 1 || row_tile > 64) return MFM_ERR_ARG;
   c->walk_tile = row_tile;
   c->walk_xcd = xcd_order ? 1 : 0;
+  return MFM_OK;
+}
+
+int mfm_ctx_set_stage_kernel(mfm_ctx* c, int fused, int tz) {
+  if (!c || tz < 1 || tz > 4096) return MFM_ERR_ARG;
+  c->fused = fused != 0;
+  c->stage_tz = tz;
   return MFM_OK;
 }
 

@@ -63,6 +63,9 @@ struct Consts {
       lism_bz, net_rho, net_p, tmlim, reg_safe_z_top, reg_safe_z_bot;
   double ce_scale_vel, ce_sigma, c13493, c0531;
   int ce_on;
+  int ce_mf;          // CHARGE_EXCHANGE_MF: neutral k is born where REG == k
+  int axisym;         // coord_sys = 1: geometric sources, r-term of div(B)
+  double r0_dt_ignore;  // SW/LISM: the dt exclusion radius
   // uniform factors of HELIOREINIT_DEFAULT, formed once on the host
   double sin_tilt, cos_tilt, sun_beta;
 };

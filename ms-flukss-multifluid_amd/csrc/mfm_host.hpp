@@ -35,6 +35,11 @@ struct mfm_ctx {
   // work walk of the region kernels (row tile in (y, z), XCD renumbering)
   int walk_tile = 8;
   int walk_xcd = 1;
+  // RK2 / first-order stages as one fused kernel (k_stage) per stage, with
+  // z-chunks of stage_tz planes per block; off (default): the separate flux
+  // and update kernels
+  bool fused = false;
+  int stage_tz = 16;
   // host staging of the allocation-free entry point (tables and results of
   // one patch); part of the context, so that entry point needs no allocation
   static constexpr size_t kFixedHostBytes = 64 * 1024;

@@ -103,6 +103,45 @@ __global__ __launch_bounds__(kB) void k_init(Consts C, double* U, Geo G,
       const int s = C.iRhoN + 10;
       u[s] = kNet3Dens; u[s + 4] = kNet3Pres / hg;
     }
+  } else if (C.problem == MFM_PROBLEM_SWLISM) {
+    // SWLISMINIT (swlism/SWLISMF.ChF:106-255), 2-D: LISM potential flow
+    // outside initR, radial solar wind inside
+    const double x = (i + 0.5) * dx - C.sun_xc, y = (j + 0.5) * dx - C.sun_yc;
+    const double r = sqrt(x * x + y * y);
+    const int s = C.iRhoN;
+    if (r > C.init_r) {
+      const double sinT = y / r, cosT = x / r;
+      double h = C.init_r / r;
+      h = h * h * h;
+      const double ux = 1.0 - (1.0 - 1.5 * sinT * sinT) * h;
+      const double uy = -1.5 * sinT * cosT * h;
+      const double uz = 0.0;
+      const double p = 1.0 / (C.gamma * C.lism_m * C.lism_m);
+      const double Bx = C.lism_bx * ux, By = C.lism_bx * uy, Bz = 0.0;
+      const double b = k1_4PI * (Bx * Bx + By * By + Bz * Bz);
+      const double ke = ux * ux + uy * uy + uz * uz;
+      u[0] = 1.0; u[1] = ux; u[2] = uy; u[3] = uz;
+      u[4] = p / hg + 0.5 * (ke + b);
+      u[5] = Bx; u[6] = By; u[7] = Bz;
+      if (C.nf > 1) {
+        u[s] = C.net_rho; u[s + 1] = C.net_rho * ux; u[s + 2] = C.net_rho * uy;
+        u[s + 3] = C.net_rho * uz;
+        u[s + 4] = C.net_p / hg + 0.5 * C.net_rho * ke;
+      }
+    } else {
+      const double rho = C.sun_rho / (r * r);
+      const double ux = C.sun_v * x / r, uy = C.sun_v * y / r;
+      const double p = C.sun_p / pow(r, 2.0 * C.gamma);
+      u[0] = rho; u[1] = rho * ux; u[2] = rho * uy; u[3] = 0.0;
+      u[4] = p / hg + 0.5 * rho * C.sun_v * C.sun_v;
+      u[5] = 0.0; u[6] = 0.0; u[7] = 0.0;
+      if (C.nf > 1) {
+        u[s] = C.net_rho; u[s + 1] = C.net_rho * 1.0; u[s + 2] = 0.0; u[s + 3] = 0.0;
+        u[s + 4] = C.net_p / hg + 0.5 * C.net_rho;
+      }
+    }
+    if (C.nf > 2) { u[s + 5] = kNet2Dens; u[s + 9] = kNet2Pres / hg; }
+    if (C.nf > 3) { u[s + 10] = kNet3Dens; u[s + 14] = kNet3Pres / hg; }
   } else if (C.problem == MFM_PROBLEM_RIEMANN) {
     const double* L = C.rpL;
     const double* R = C.rpR;

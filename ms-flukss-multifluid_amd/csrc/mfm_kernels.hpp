@@ -15,6 +15,9 @@ struct DevPatch {
   double* Up;       // first-stage state U* (scratch)
   void* W;          // primitive variables (scratch, flux precision)
   void* W2;         // Hancock: primitives of the predicted state
+  void* Wn;         // where the stage writes W of its output (the next
+                    // stage's W): W itself, or a second buffer when the
+                    // fused stage kernel still reads W from other blocks
   void* F[3];       // face fluxes per direction (scratch, flux precision)
   void* Bn[3];      // face-normal B per direction (scratch)
   double* Fo[3];    // optional time-averaged fluxes returned to the caller
@@ -51,6 +54,8 @@ struct RegionList {
 // a physical (non-periodic) domain boundary.
 struct StageLists {
   RegionList prim, flux[3], fbc[3], upd;
+  int fused;        // upd is a list of k_stage tiles (interior faces and the
+                    // update in one launch); flux[] is then unused
 };
 
 struct StepArgs {
@@ -66,12 +71,13 @@ struct StepArgs {
   int xcd_order;       // work walk: 1 = renumber blocks per XCD
   int hancock;         // 0: Euler / RK2 stage; 1: Hancock predictor;
                        // 2: Hancock corrector
+  int tz;              // k_stage: planes of one z-chunk of a tile
 };
 
 // Optional kernel timing: a pair of events brackets every launch on the
 // launch stream; the host reads them after the step has synchronized.
 enum KernelKind { KK_PRIM = 0, KK_FLUX_PLASMA, KK_FLUX_NEUTRAL, KK_FLUX_BC, KK_UPDATE,
-                  KK_PREDICT, KK_COUNT };
+                  KK_PREDICT, KK_STAGE, KK_COUNT };
 struct KernelTimer {
   static constexpr int kMax = 32;
   hipEvent_t ev[2 * kMax];
@@ -95,6 +101,13 @@ template <typename TF>
 hipError_t launch_stage(const Consts& C, const Consts* CT, const DevPatch* P,
                         const StageLists& L, const StepArgs& A, double* dt_part,
                         long* idx_part, hipStream_t s, KernelTimer* T);
+
+// The fused stage kernel (k_stage) over L.upd; launch_stage has already
+// run k_prim and the boundary-face fluxes of the stage.
+template <typename TF>
+hipError_t launch_stage_fused(const Consts& C, const Consts* CT, const DevPatch* P,
+                              const StageLists& L, const StepArgs& A, double* dt_part,
+                              long* idx_part, hipStream_t s, KernelTimer* T);
 
 hipError_t launch_dt_final(const RegionList& L, const double* dt_part,
                            const long* idx_part, double* dt_out, long* idx_out,

@@ -89,7 +89,8 @@ size_t request_bytes(const mfm_params& cls, const Consts& C, const HBox& fab, co
 int solver(const mfm_params& p) { return p.dedner ? 2 : ((p.flux == 2 || p.flux == 4) ? 1 : 0); }
 
 bool same_class(const mfm_params& a, const mfm_params& b) {
-  return a.spacedim == b.spacedim && a.nfluids == b.nfluids && a.dedner == b.dedner &&
+  return a.spacedim == b.spacedim && a.coord_sys == b.coord_sys &&
+         a.nfluids == b.nfluids && a.dedner == b.dedner &&
          solver(a) == solver(b) &&
          a.time_approx == b.time_approx && a.precision == b.precision &&
          (a.char_reconstruction != 0) == (b.char_reconstruction != 0);

@@ -1,0 +1,10 @@
+// FP64 (parity) build of the fused stage kernel, -ffp-contract=off as
+// mfm_kernels_fp64.hip: every product and sum rounds as in the reference.
+#include "mfm_kernels.inc"
+#include "mfm_stage.inc"
+
+namespace mfm {
+template hipError_t launch_stage_fused<double>(const Consts&, const Consts*, const DevPatch*,
+                                               const StageLists&, const StepArgs&, double*,
+                                               long*, hipStream_t, KernelTimer*);
+}  // namespace mfm

@@ -23,7 +23,7 @@ FLUX_ROE8, FLUX_RUSANOV, FLUX_ROE8_ROEAVG, FLUX_RUSANOV_ROEAVG = 1, 2, 3, 4
  LIM_NONE) = range(13)
 TIME_FIRST_ORDER, TIME_HANCOCK, TIME_RK2 = 0, 1, 2
 COORD_CARTESIAN, COORD_AXISYMMETRIC = 0, 1
-PROBLEM_NONE, PROBLEM_RIEMANN, PROBLEM_HELIO = 0, 1, 2
+PROBLEM_NONE, PROBLEM_RIEMANN, PROBLEM_HELIO, PROBLEM_SWLISM = 0, 1, 2, 3
 PREC_FP64, PREC_FP32_FLUX = 0, 1
 ABI_VERSION = 2
 
@@ -88,7 +88,8 @@ class PatchRec(C.Structure):
 
 EXPORTS = [
     "mfm_params_sizeof", "mfm_params_default", "mfm_params_set_riemann",
-    "mfm_params_set_orszag", "mfm_params_set_helio", "mfm_num_components",
+    "mfm_params_set_orszag", "mfm_params_set_helio", "mfm_params_set_swlism",
+    "mfm_num_components",
     "mfm_num_ghost_cells", "mfm_ctx_create", "mfm_ctx_destroy",
     "mfm_ctx_set_dedner", "mfm_ctx_last_error", "mfm_patch_update_dev",
     "mfm_patch_update_noalloc_dev",
@@ -99,6 +100,7 @@ EXPORTS = [
     "mfm_event_destroy", "mfm_event_record", "mfm_event_elapsed_ms",
     "mfm_ctx_set_timing", "mfm_ctx_get_timing", "mfm_ctx_set_graphs",
     "mfm_ctx_set_walk", "mfm_ctx_get_walk", "mfm_ctx_autotune",
+    "mfm_ctx_set_stage_kernel",
     "mfm_server_create", "mfm_server_destroy", "mfm_server_last_error",
     "mfm_server_submit", "mfm_server_step", "mfm_server_request_info",
     "mfm_server_fetch", "mfm_server_release", "mfm_server_stats",
@@ -121,9 +123,10 @@ EXPORTS = [
 REQ_PENDING, REQ_ACTIVE, REQ_DONE, REQ_FAILED = range(4)
 
 # kernel kinds of mfm_ctx_get_timing
-KT_PRIM, KT_FLUX_PLASMA, KT_FLUX_NEUTRAL, KT_FLUX_BC, KT_UPDATE, KT_PREDICT = range(6)
+KT_PRIM, KT_FLUX_PLASMA, KT_FLUX_NEUTRAL, KT_FLUX_BC, KT_UPDATE, KT_PREDICT, KT_STAGE = \
+    range(7)
 KT_NAMES = ("k_prim", "k_flux_plasma", "k_flux_neutral", "k_flux_bc", "k_update",
-            "k_hancock_pred")
+            "k_hancock_pred", "k_stage")
 
 _lib = None
 
@@ -150,6 +153,10 @@ def load(path=LIB_PATH):
         dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
         C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
         ip]
+    lib.mfm_params_set_swlism.argtypes = [
+        P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, dp,
+        C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+        C.c_double, C.c_double, ip]
     lib.mfm_num_components.argtypes = [P]
     lib.mfm_num_ghost_cells.argtypes = [P]
     lib.mfm_ctx_create.argtypes = [P, C.POINTER(C.c_void_p)]
@@ -299,6 +306,24 @@ def helio_params(gamma=5.0 / 3.0, lismN=0.06, lismV=2.64e6, lismT=6527.0,
     return p
 
 
+def swlism_params(gamma=5.0 / 3.0, lismN=0.06, lismV=2.64e6, lismT=6527.0,
+                  lismB=1.5e-6, sunXY=(600.0, 0.0), R0=10.0, sunN=7.4, sunV=4.5e7,
+                  sunT=51100.0, initR=80.0, netN=0.1, TMLIM=50000.0, dx=15.625,
+                  num_cells=(96, 64), nfluids=2):
+    """SWLISMProblem input (MHDAM-3.2.4/swlism/SWLISMProblem.cpp:68-256)
+    with the reference deck's values (main/inputs/swlism2F1.inputs):
+    2-D axisymmetric, lengths in AU, dx = domain_length / num_cells[0]."""
+    p = default_params()
+    nc = (C.c_int * 2)(*[int(v) for v in num_cells])
+    rc = load().mfm_params_set_swlism(
+        C.byref(p), gamma, lismN, lismV, lismT, lismB, _darr(sunXY), R0, sunN, sunV,
+        sunT, initR, netN, TMLIM, dx, nc)
+    if rc != OK:
+        raise MfmError(rc, "mfm_params_set_swlism failed")
+    p.nfluids = nfluids
+    return p
+
+
 def num_components(p):
     return load().mfm_num_components(C.byref(p))
 
@@ -378,6 +403,11 @@ class Context:
 
     def set_walk(self, row_tile, xcd_order):
         self._check(self.lib.mfm_ctx_set_walk(self.h, int(row_tile), int(xcd_order)))
+
+    def set_stage_kernel(self, fused, tz=16):
+        """fused=True: one k_stage launch per stage (interior faces + update);
+        False: the separate flux kernels and k_update (same results)."""
+        self._check(self.lib.mfm_ctx_set_stage_kernel(self.h, 1 if fused else 0, int(tz)))
 
     def walk(self):
         t, x = C.c_int(), C.c_int()

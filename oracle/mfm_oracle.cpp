@@ -1394,9 +1394,22 @@ void charge_exchange_cell(const Eq& E, const double* w, int REG, double dt,
                             0.75 * VVTH[f] * UPE[f]);
     // neutral 1 is born in region 1 (and 4 in the 2- and 4-fluid kernels,
     // CHARGE_EXCHANGE_3F tests REG == 1 only, :618); neutral k in region k
-    bool born = (f == 0) ? (REG == 1 || (REG == 4 && E.nf != 3)) : (REG == f + 1);
+    // CHARGE_EXCHANGE_MF (:995-1149, the SW/LISM problem's kernel): neutral
+    // k is born where REG == k
+    // (SWLISMProblem::explicitSource, swlism/SWLISMProblem.cpp:945-1021,
+    // calls it for 2 and 3 fluids and CHARGE_EXCHANGE_4F for 4)
+    const bool mf = E.P->problem == MFM_PROBLEM_SWLISM && E.nf != 4;
+    bool born = mf ? (REG == f + 1)
+                   : ((f == 0) ? (REG == 1 || (REG == 4 && E.nf != 3)) : (REG == f + 1));
     double H1, H2, H3, H4, H5;
-    if (born) {
+    if (born && mf) {
+      // sum(DENPLU) - DENPLU(iFluid), the sum in index order
+      double sum = zero;
+      for (int g = 0; g < nn; ++g) sum = sum + DENPLU[g];
+      H1 = sum - DENPLU[f];
+      H2 = PPLU[0] + PMI[0]; H3 = PPLU[1] + PMI[1]; H4 = PPLU[2] + PMI[2];
+      H5 = ENPLU + ENMI;
+    } else if (born) {
       // sum of the other fluids' DENPLU in index order (zero if none)
       bool first = true;
       H1 = zero;
@@ -1416,6 +1429,255 @@ void charge_exchange_cell(const Eq& E, const double* w, int REG, double dt,
     s[b] = -dt * H1;
     s[b + 1] = -dt * H2; s[b + 2] = -dt * H3; s[b + 3] = -dt * H4;
     s[b + 4] = -dt * H5;
+  }
+}
+
+// ================================================================ SW/LISM ==
+// The 2-D axisymmetric solar-wind / LISM problem (swlism/SWLISMProblem.cpp,
+// swlism/SWLISMF.ChF) in coord_sys = 1 (CS_Axisymmetric: Cartesian fluxes
+// and volumes, src/CSHCartesian.cpp:254-305, plus the geometric sources).
+
+// LISMINIT (swlism/SWLISMF.ChF:512-652): the LISM potential flow past the
+// sphere initR on a ghost box; populations 2.. copy the interior cell next
+// to the boundary (right / left boundary in x, top in y).
+void swlism_lisminit(const Eq& E, Fab& W, int lohisign, int dir, const Box& box, double dx) {
+  const mfm_params* P = E.P;
+  FORBOX(box, i, j, k) {
+    double x = (i + half) * dx - P->sun_xc;
+    double y = (j + half) * dx - P->sun_yc;
+    double r = std::sqrt(x * x + y * y);
+    double sinT = y / r, cosT = x / r;
+    double h = P->init_r / r;
+    h = h * h * h;
+    double ux = one - (one - 1.5 * sinT * sinT) * h;
+    double uy = -1.5 * sinT * cosT * h;
+    double Bx = P->lism_bx * ux, By = P->lism_bx * uy;
+    W(i, j, k, WRHO) = one;
+    W(i, j, k, WVELX) = ux; W(i, j, k, WVELY) = uy; W(i, j, k, WVELZ) = zero;
+    W(i, j, k, WPRES) = P->lism_p;
+    W(i, j, k, WBX) = Bx; W(i, j, k, WBY) = By; W(i, j, k, WBZ) = zero;
+  }
+  if (E.nf > 1) {
+    int s0 = E.iRhoN;
+    FORBOX(box, i, j, k) {
+      W(i, j, k, s0 + WRHO) = P->net_rho;
+      W(i, j, k, s0 + WVELX) = W(i, j, k, WVELX);
+      W(i, j, k, s0 + WVELY) = W(i, j, k, WVELY);
+      W(i, j, k, s0 + WVELZ) = W(i, j, k, WVELZ);
+      W(i, j, k, s0 + WPRES) = P->net_p;
+    }
+  }
+  if (E.nf > 2) {
+    int vb = E.iRhoN + WNUM_E, ve = E.iRhoN + (E.nf - 1) * WNUM_E - 1;
+    if (dir == 0) {
+      int indb = lohisign == 1 ? box.lo[0] - 1 : box.hi[0] + 1;
+      FORBOX(box, i, j, k) for (int v = vb; v <= ve; ++v) W(i, j, k, v) = W(indb, j, k, v);
+    } else if (lohisign == 1) {
+      int indb = box.lo[1] - 1;
+      FORBOX(box, i, j, k) for (int v = vb; v <= ve; ++v) W(i, j, k, v) = W(i, indb, k, v);
+    }
+  }
+}
+
+// SWLISMProblem::fillGhostCellsDefault (swlism/SWLISMProblem.cpp:453-635):
+// LISM inflow / outflow states on the x boundaries and the top, and the
+// mirror image across the symmetry axis (y = 0) at the bottom, with the
+// y velocities and B_y reversed.  At most 4 ghost layers.
+void swlism_fill_ghost(const Eq& E, Fab& W, int dir, double dx) {
+  if (!E.physical(dir) || dir > 1) return;
+  Box WBox = W.b;
+  Box tmp = E.clip(WBox);
+  if (WBox.hi[dir] != tmp.hi[dir]) {
+    int nGS = std::min(std::abs(WBox.hi[dir] - tmp.hi[dir]), 4);
+    Box bb = WBox; bb.lo[dir] = tmp.hi[dir] + 1; bb.hi[dir] = tmp.hi[dir] + nGS;
+    swlism_lisminit(E, W, 1, dir, bb, dx);
+  }
+  if (WBox.lo[dir] != tmp.lo[dir]) {
+    int nGS = std::min(std::abs(WBox.lo[dir] - tmp.lo[dir]), 4);
+    Box bb = WBox; bb.lo[dir] = tmp.lo[dir] - nGS; bb.hi[dir] = tmp.lo[dir] - 1;
+    if (dir == 0) {
+      swlism_lisminit(E, W, -1, dir, bb, dx);
+    } else {
+      int src = tmp.lo[1], des = tmp.lo[1] - 1;
+      for (int q = 0; q < nGS; ++q, ++src, --des) {
+        for (int k = WBox.lo[2]; k <= WBox.hi[2]; ++k)
+          for (int i = WBox.lo[0]; i <= WBox.hi[0]; ++i)
+            for (int c = 0; c < E.nc; ++c) W(i, des, k, c) = W(i, src, k, c);
+      }
+      FORBOX(bb, i, j, k) {
+        W(i, j, k, WVELY) = -W(i, j, k, WVELY);
+        W(i, j, k, WBY) = -W(i, j, k, WBY);
+        for (int f = 1; f < E.nf; ++f) {
+          int s0 = E.iRhoN + (f - 1) * WNUM_E;
+          W(i, j, k, s0 + WVELY) = -W(i, j, k, s0 + WVELY);
+        }
+      }
+    }
+  }
+}
+
+// BFLUXGAS (swlism/SWLISMF.ChF:1434-1553): characteristic outflow flux of
+// a neutral population; the subsonic branches that return early keep the
+// Riemann flux passed in.
+void swlism_bfluxgas(const Eq& E, double rho_in, double vn_in, double vt1_in,
+                     double vt2_in, double p_in, double rho_out, double vn_out,
+                     double vt1_out, double vt2_out, double p_out, double* F5) {
+  const double BET = E.hgamma, GAM = E.gamma, GAL = E.invhgamma;
+  double UBO = vn_in, VBO = vt1_in, WBO = vt2_in;
+  double RBO = std::max(E.smallr, rho_in), PBO = std::max(E.smallp, p_in);
+  double KEBO = half * (vn_in * vn_in + vt1_in * vt1_in + vt2_in * vt2_in);
+  double EBO = PBO / E.hgamma + rho_in * KEBO;
+  double SZBO = std::sqrt(GAM * PBO / RBO);
+  double U1 = vn_out, V1 = vt1_out, W1 = vt2_out;
+  double R1 = std::max(E.smallr, rho_out), P1 = std::max(E.smallp, p_out);
+  double KE1 = half * (vn_out * vn_out + vt1_out * vt1_out + vt2_out * vt2_out);
+  double E1 = P1 / E.hgamma + rho_out * KE1;
+  double SZ1 = std::sqrt(GAM * P1 / R1);
+  double EN = PBO / std::pow(RBO, GAM), EN1 = P1 / std::pow(R1, GAM);
+  double U16 = BET * (UBO + 2.0 * SZBO / BET) / (GAM + 1.0);
+  double SZ16 = U16;
+  double R16 = std::pow(SZ16 * SZ16 / (GAM * EN), GAL);
+  double V16 = VBO, W16 = WBO;
+  double P16 = SZ16 * SZ16 * R16 / GAM;
+  double E16 = P16 * GAL + .5 * R16 * (U16 * U16 + V16 * V16 + W16 * W16);
+  double U56 = BET * (U1 - 2.0 * SZ1 / BET) / (GAM + 1.0);
+  double SZ56 = -U56;
+  double R56 = std::pow(SZ56 * SZ56 / (GAM * EN1), GAL);
+  double V56 = V1, W56 = W1;
+  double P56 = SZ56 * SZ56 * R56 / GAM;
+  double E56 = P56 * GAL + .5 * R56 * (U56 * U56 + V56 * V56 + W56 * W56);
+  if ((UBO - SZBO) >= 0.0) {
+    F5[0] = RBO * UBO; F5[1] = RBO * UBO * UBO + PBO; F5[2] = RBO * UBO * VBO;
+    F5[3] = RBO * UBO * WBO; F5[4] = (EBO + PBO) * UBO;
+    return;
+  }
+  if (((UBO - SZBO) < 0.0) && (UBO >= 0.0)) {
+    if ((U1 - SZ1) <= 0.0) return;
+    F5[0] = R16 * U16; F5[1] = R16 * U16 * U16 + P16; F5[2] = R16 * U16 * V16;
+    F5[3] = R16 * U16 * W16; F5[4] = (E16 + P16) * U16;
+    return;
+  }
+  if ((UBO < 0.0) && (UBO + SZBO) > 0.0) {
+    if ((U1 + SZ1) >= 0.0) return;
+    F5[0] = R56 * U56; F5[1] = R56 * U56 * U56 + P56; F5[2] = R56 * U56 * V56;
+    F5[3] = R56 * U56 * W56; F5[4] = (E56 + P56) * U56;
+    return;
+  }
+  if ((UBO + SZBO) < 0.0) {
+    F5[0] = R1 * U1; F5[1] = R1 * U1 * U1 + P1; F5[2] = R1 * U1 * V1;
+    F5[3] = R1 * U1 * W1; F5[4] = (E1 + P1) * U1;
+  }
+}
+
+// SWLISMProblem::fluxBC (swlism/SWLISMProblem.cpp:727-862) at one boundary
+// face, after the Riemann fluxes of the face states are in out:
+// SWLISMBC (swlism/SWLISMF.ChF:1025-1205) -- BFLUX for the plasma and
+// BFLUXGAS for the neutrals on the right and top boundaries, the mean normal
+// field on the left -- and a zero normal field on the axis.
+void swlism_bc_face(const Eq& E, int lohisign, int dir, const double* wl,
+                    const double* wr, double* out, double* bn) {
+  Rot r = rot(0, dir);
+  if (lohisign == 1) {
+    double FL[8], BnValue;
+    // SWLISMBC passes the face states unclamped (BFLUX, HeliosphericF.ChF:1363)
+    helio_bflux(E, false, wl[WRHO], wl[r.n], wl[r.t1], wl[r.t2], wl[WPRES], wl[r.bn],
+                wl[r.bt1], wl[r.bt2], wr[WRHO], wr[r.n], wr[r.t1], wr[r.t2], wr[WPRES],
+                wr[r.bn], wr[r.bt1], wr[r.bt2], zero, zero, zero, FL, &BnValue);
+    out[URHO] = FL[0]; out[r.n] = FL[1]; out[r.t1] = FL[2]; out[r.t2] = FL[3];
+    out[UENG] = FL[4]; out[r.bn] = FL[5]; out[r.bt1] = FL[6]; out[r.bt2] = FL[7];
+    *bn = BnValue;
+    for (int f = 1; f < E.nf; ++f) {
+      int s0 = E.iRhoN + (f - 1) * WNUM_E;
+      Rot q = rot(s0, dir);
+      double F5[5] = {out[q.rho], out[q.n], out[q.t1], out[q.t2], out[q.p]};
+      swlism_bfluxgas(E, wl[q.rho], wl[q.n], wl[q.t1], wl[q.t2], wl[q.p], wr[q.rho],
+                      wr[q.n], wr[q.t1], wr[q.t2], wr[q.p], F5);
+      out[q.rho] = F5[0]; out[q.n] = F5[1]; out[q.t1] = F5[2]; out[q.t2] = F5[3];
+      out[q.p] = F5[4];
+    }
+  } else if (dir == 0) {
+    *bn = half * (wl[r.bn] + wr[r.bn]);
+  } else {
+    *bn = zero;  // the symmetry axis (swlism/SWLISMProblem.cpp:854)
+  }
+}
+
+// SWLISMREINIT (swlism/SWLISMF.ChF:305-403): the supersonic solar wind
+// inside R0 (early exit on the FAB bounds, as the reference).
+void swlism_reinit(const Eq& E, Fab& U, double dx, const Box& box) {
+  const mfm_params* P = E.P;
+  double sunR = P->sun_r0;
+  double xl = U.b.lo[0] * dx - P->sun_xc, yl = U.b.lo[1] * dx - P->sun_yc;
+  double xh = (U.b.hi[0] + 1) * dx - P->sun_xc, yh = (U.b.hi[1] + 1) * dx - P->sun_yc;
+  if ((xl > sunR) || (yl > sunR)) return;
+  if ((xh < -sunR) || (yh < -sunR)) return;
+  double sunR2 = sunR * sunR;
+  FORBOX(box, i, j, k) {
+    double x = (i + half) * dx - P->sun_xc;
+    double y = (j + half) * dx - P->sun_yc;
+    double r2 = x * x + y * y;
+    if (r2 <= sunR2) {
+      double r = std::sqrt(r2);
+      double rho = P->sun_rho / (r * r);
+      double ux = P->sun_v * x / r, uy = P->sun_v * y / r;
+      double p = P->sun_p / std::pow(r, two * E.gamma);
+      double e0 = p / E.hgamma + half * rho * P->sun_v * P->sun_v;
+      U(i, j, k, URHO) = rho;
+      U(i, j, k, UMOMX) = rho * ux; U(i, j, k, UMOMY) = rho * uy; U(i, j, k, UMOMZ) = zero;
+      U(i, j, k, UENG) = e0;
+      U(i, j, k, UBX) = zero; U(i, j, k, UBY) = zero; U(i, j, k, UBZ) = zero;
+      if (E.nf > 2) {
+        int s0 = E.iRhoN + WNUM_E;
+        U(i, j, k, s0 + URHO) = netFluid2Dens;
+        U(i, j, k, s0 + UMOMX) = zero; U(i, j, k, s0 + UMOMY) = zero;
+        U(i, j, k, s0 + UMOMZ) = zero;
+        U(i, j, k, s0 + UENG) = netFluid2Pres / E.hgamma;
+      }
+      if (E.nf > 3) {
+        int s0 = E.iRhoN + 2 * WNUM_E;
+        U(i, j, k, s0 + URHO) = netFluid3Dens;
+        U(i, j, k, s0 + UMOMX) = zero; U(i, j, k, s0 + UMOMY) = zero;
+        U(i, j, k, s0 + UMOMZ) = zero;
+        U(i, j, k, s0 + UENG) = netFluid3Pres / E.hgamma;
+      }
+    }
+  }
+}
+
+// Geometric sources of coord_sys = 1: SOURCEAXISYMMETRIC
+// (srcIdeal/PatchIdealMHDF.ChF:839-903) and, with neutral fluids,
+// SOURCEAXISYMMETRIC_MF (srcIdeal/PatchMHDMFF.ChF:152-244) of one cell;
+// s[0 .. nc-1] (zero where the reference writes zero).
+void axisym_source_cell(const Eq& E, const double* w, int j, double dt, double dx, double* s) {
+  double y = dx * (j + half);
+  double h = -dt / y;
+  double Rho = w[WRHO], Ux = w[WVELX], Uy = w[WVELY], Uz = w[WVELZ], P = w[WPRES];
+  double Bx = w[WBX], By = w[WBY], Bz = w[WBZ];
+  double By4Pi = d_1_4PI * By;
+  double BxB = d_1_4PI * (Bx * Bx + By * By + Bz * Bz);
+  double UxU = half * (Ux * Ux + Uy * Uy + Uz * Uz);
+  double e = P / E.hgamma + Rho * UxU;
+  for (int c = 0; c < E.nc; ++c) s[c] = zero;
+  s[URHO] = h * Rho * Uy;
+  s[UMOMX] = h * (Rho * Ux * Uy - By4Pi * Bx);
+  s[UMOMY] = h * (Rho * Uy * Uy - By4Pi * By);
+  s[UMOMZ] = zero;
+  s[UENG] = h * (Uy * (e + P + BxB) - By4Pi * (Bx * Ux + By * Uy));
+  s[UBX] = h * (Uy * Bx - Ux * By);
+  s[UBY] = zero;
+  s[UBZ] = zero;
+  for (int f = 1; f < E.nf; ++f) {
+    int sh = E.iRhoN + (f - 1) * WNUM_E;
+    double RhoN = w[sh + WRHO], UxN = w[sh + WVELX], UyN = w[sh + WVELY],
+           UzN = w[sh + WVELZ], PN = w[sh + WPRES];
+    double UxUN = half * (UxN * UxN + UyN * UyN + UzN * UzN);
+    double eN = PN / E.hgamma + RhoN * UxUN;
+    double hu = h * UyN;
+    s[sh + URHO] = hu * RhoN;
+    s[sh + UMOMX] = hu * RhoN * UxN;
+    s[sh + UMOMY] = hu * RhoN * UyN;
+    s[sh + UMOMZ] = zero;
+    s[sh + UENG] = hu * (eN + PN);
   }
 }
 
@@ -1585,9 +1847,10 @@ struct FaceSet {
   Fab Bn[3];
 };
 
-void fill_ghost(const Eq& E, Fab& W, int dir) {
+void fill_ghost(const Eq& E, Fab& W, int dir, double dx) {
   if (E.P->problem == MFM_PROBLEM_RIEMANN) riemann_fill_ghost(E, W, dir);
   else if (E.P->problem == MFM_PROBLEM_HELIO) helio_fill_ghost(E, W, dir);
+  else if (E.P->problem == MFM_PROBLEM_SWLISM) swlism_fill_ghost(E, W, dir, dx);
 }
 
 // ========================================= characteristic reconstruction ==
@@ -1900,14 +2163,15 @@ void euler_step(const Eq& E, Fab& U, Fab* Fout, Fab* divBout, double dt,
   Fab W;
   W.define(UBox, E.nc, std::numeric_limits<double>::quiet_NaN());
   cons_to_prim(E, W, U, WBoxB);
-  for (int d = 0; d < dims; ++d) fill_ghost(E, W, d);
+  for (int d = 0; d < dims; ++d) fill_ghost(E, W, d, dx);
 
   Box valueBox = E.clip(box);
   Box slopeBox = grow(valueBox, 1, dims);
 
   // preprocessing -> defineRegions (srcIdeal/PatchMHDMF.cpp:1447-1458)
   IFab REG;
-  bool ce = (E.nf >= 2 && E.P->problem == MFM_PROBLEM_HELIO);
+  bool ce = (E.nf >= 2 && (E.P->problem == MFM_PROBLEM_HELIO ||
+                           E.P->problem == MFM_PROBLEM_SWLISM));
   if (ce) {
     REG.define(slopeBox);
     FORBOX(slopeBox, i, j, k) {
@@ -2007,6 +2271,12 @@ void euler_step(const Eq& E, Fab& U, Fab* Fout, Fab* divBout, double dt,
             for (int f = 1; f < E.nf; ++f)
               neutral_flux_face(E, wl, wr, d, E.iRhoN + (f - 1) * WNUM_E, out);
             helio_bc_face(E, side == 0 ? -1 : 1, d, dx, i, j, k, wl, wr, out, &bn);
+          } else if (E.P->problem == MFM_PROBLEM_SWLISM) {
+            // m_RS->fluxes + m_RSGD->fluxes, then SWLISMBC
+            plasma_flux_face(E, wl, wr, d, out);
+            for (int f = 1; f < E.nf; ++f)
+              neutral_flux_face(E, wl, wr, d, E.iRhoN + (f - 1) * WNUM_E, out);
+            swlism_bc_face(E, side == 0 ? -1 : 1, d, wl, wr, out, &bn);
           }
           for (int c = 0; c < E.nc; ++c) F(i, j, k, c) = out[c];
           Bn[d](i, j, k, 0) = bn;
@@ -2038,6 +2308,14 @@ void euler_step(const Eq& E, Fab& U, Fab* Fout, Fab* divBout, double dt,
                          Bn[d](i, j, k, 0);
     }
   }
+  const bool axisym = E.P->coord_sys == MFM_COORD_AXISYMMETRIC;
+  if (axisym) {  // AXISYMMETRICDBZBYDZ (srcIdeal/PatchIdealMHDF.ChF:661-679)
+    FORBOX(valueBox, i, j, k) {
+      double y = dx * (j + half);
+      double By = W(i, j, k, WBY);
+      divB(i, j, k, 0) = divB(i, j, k, 0) + dx * (By / y);
+    }
+  }
   if (E.P->eight_wave) {  // SOURCE8WAVES :681-722 + ADDSOURCES UMOMX..UBZ
     double dtbydx = dt / dx;
     PFORBOX(valueBox, i, j, k) {
@@ -2061,6 +2339,16 @@ void euler_step(const Eq& E, Fab& U, Fab* Fout, Fab* divBout, double dt,
       }
     }
   }
+  if (axisym) {
+    // geometric sources (srcIdeal/PatchMHDMF.cpp:1237-1276) + ADDSOURCES
+    // from URHO to UBZ (one fluid) or the last neutral's UENG
+    PFORBOX(valueBox, i, j, k) {
+      double w[32], sa[32];
+      for (int c = 0; c < E.nc; ++c) w[c] = W(i, j, k, c);
+      axisym_source_cell(E, w, j, dt, dx, sa);
+      for (int c = 0; c < E.nc; ++c) U(i, j, k, c) = U(i, j, k, c) + sa[c];
+    }
+  }
   if (E.dedner) {  // PSIPARABOLIC (src/DednerF.ChF:65-87)
     double coeff = std::exp(-dt * E.ch * E.ch / (E.cp * E.cp));
     FORBOX(valueBox, i, j, k) U(i, j, k, E.iPsi) = coeff * U(i, j, k, E.iPsi);
@@ -2077,6 +2365,7 @@ void euler_step(const Eq& E, Fab& U, Fab* Fout, Fab* divBout, double dt,
   }
   // problem postprocessing (HeliosphericProblem::postprocessing :2399-2476)
   if (E.P->problem == MFM_PROBLEM_HELIO) helio_reinit(E, U, dx, dt + time, valueBox);
+  if (E.P->problem == MFM_PROBLEM_SWLISM) swlism_reinit(E, U, dx, valueBox);
   // PatchMHDMF::postprocessing (srcIdeal/PatchMHDMF.cpp:1392-1440)
   PFORBOX(valueBox, i, j, k) {
     double u[32];
@@ -2226,7 +2515,7 @@ void hancock_step(const Eq& E, Fab& U, Fab* Fout, Fab* divBout, double dt, doubl
   Fab W;
   W.define(UBox, E.nc, std::numeric_limits<double>::quiet_NaN());
   cons_to_prim(E, W, UP, WBoxB);
-  for (int d = 0; d < dims; ++d) fill_ghost(E, W, d);
+  for (int d = 0; d < dims; ++d) fill_ghost(E, W, d, dx);
   const Box valueP = E.clip(grow(box, 1, dims));
   const Box slopeBox = grow(valueP, 1, dims);
   IFab REG;
@@ -2275,7 +2564,7 @@ void hancock_step(const Eq& E, Fab& U, Fab* Fout, Fab* divBout, double dt, doubl
   Fab WPred;
   WPred.define(UBox, E.nc, std::numeric_limits<double>::quiet_NaN());
   cons_to_prim(E, WPred, UP, WBoxB);
-  for (int d = 0; d < dims; ++d) fill_ghost(E, WPred, d);
+  for (int d = 0; d < dims; ++d) fill_ghost(E, WPred, d, dx);
   for (int d = 0; d < dims; ++d) {  // UPDATEVALUESONFACES (src/PatchMHDAMF.ChF:640-661)
     for (int c = 0; c < E.nc; ++c) {
       FORBOX(slopeBox, i, j, k) {
@@ -2376,7 +2665,12 @@ double compute_dt(const Eq& E, const Fab& U, const Box& box, double dx,
                   int cell[3]) {
   double dtMin = std::numeric_limits<double>::max();
   bool first = true;
-  double ign = E.P->problem == MFM_PROBLEM_HELIO ? E.P->sun_r0 - 2.0 * dx : -1.0;
+  // the inner-boundary exclusion of HeliosphericProblem::computeDt (R0 - 2dx
+  // of the level) and SWLISMProblem::computeDt (m_R0dtIgnore = R0 - dx of
+  // the coarsest level, swlism/SWLISMProblem.cpp:153, 1539-1582)
+  double ign = E.P->problem == MFM_PROBLEM_HELIO    ? E.P->sun_r0 - 2.0 * dx
+               : E.P->problem == MFM_PROBLEM_SWLISM ? E.P->r0_dt_ignore
+                                                    : -1.0;
   double ign2 = ign * ign;
   cell[0] = box.lo[0]; cell[1] = box.lo[1]; cell[2] = box.lo[2];
   FORBOX(box, i, j, k) {
@@ -2410,7 +2704,15 @@ int check_args(const mfm_params* p) {
     return MFM_ERR_UNSUPPORTED;
   // FLUXESHANCOCK has no psi flux: the GLM system is not run with Hancock
   if (p->time_approx == MFM_TIME_HANCOCK && p->dedner) return MFM_ERR_UNSUPPORTED;
-  if (p->coord_sys != MFM_COORD_CARTESIAN) return MFM_ERR_UNSUPPORTED;
+  // coord_sys = 1 with the 2-D SW/LISM problem (RK2 or first order, no GLM)
+  if (p->coord_sys != MFM_COORD_CARTESIAN &&
+      !(p->coord_sys == MFM_COORD_AXISYMMETRIC && p->spacedim == 2 && !p->dedner &&
+        p->time_approx != MFM_TIME_HANCOCK &&
+        (p->problem == MFM_PROBLEM_SWLISM || p->problem == MFM_PROBLEM_NONE)))
+    return MFM_ERR_UNSUPPORTED;
+  if (p->problem == MFM_PROBLEM_SWLISM &&
+      (p->spacedim != 2 || p->dedner || p->time_approx == MFM_TIME_HANCOCK))
+    return MFM_ERR_UNSUPPORTED;
   if (p->check_tvd < 0 || p->check_tvd > 2) return MFM_ERR_ARG;
   if (p->char_reconstruction)  // no first-order (unlimited) plasma slopes
     for (int v = 0; v < 8; ++v)
@@ -2616,6 +2918,69 @@ int oracle_init_problem(const mfm_params* p, mfm_fab Uf, mfm_box bx, double dx) 
     }
     return MFM_OK;
   }
+  if (p->problem == MFM_PROBLEM_SWLISM) {
+    // SWLISMINIT (swlism/SWLISMF.ChF:106-255): the LISM potential flow past
+    // the sphere initR outside it, the radial solar wind inside
+    const int nf = E.nf, iRhoN = E.iRhoN;
+    FORBOX(box, i, j, k) {
+      double x = (i + half) * dx - p->sun_xc, y = (j + half) * dx - p->sun_yc;
+      double r = std::sqrt(x * x + y * y);
+      if (r > p->init_r) {
+        double sinT = y / r, cosT = x / r;
+        double h = p->init_r / r;
+        h = h * h * h;
+        double ux = one - (one - 1.5 * sinT * sinT) * h;
+        double uy = -1.5 * sinT * cosT * h;
+        double uz = zero;
+        double pr = one / (E.gamma * p->lism_m * p->lism_m);
+        double Bx = p->lism_bx * ux, By = p->lism_bx * uy, Bz = zero;
+        double b = d_1_4PI * (Bx * Bx + By * By + Bz * Bz);
+        double ke = ux * ux + uy * uy + uz * uz;
+        double e0 = pr / hgamma + half * (ke + b);
+        U(i, j, k, URHO) = one;
+        U(i, j, k, UMOMX) = ux; U(i, j, k, UMOMY) = uy; U(i, j, k, UMOMZ) = uz;
+        U(i, j, k, UENG) = e0;
+        U(i, j, k, UBX) = Bx; U(i, j, k, UBY) = By; U(i, j, k, UBZ) = Bz;
+        if (nf > 1) {
+          e0 = p->net_p / hgamma + half * p->net_rho * ke;
+          U(i, j, k, iRhoN + URHO) = p->net_rho;
+          U(i, j, k, iRhoN + UMOMX) = p->net_rho * ux;
+          U(i, j, k, iRhoN + UMOMY) = p->net_rho * uy;
+          U(i, j, k, iRhoN + UMOMZ) = p->net_rho * uz;
+          U(i, j, k, iRhoN + UENG) = e0;
+        }
+      } else {
+        double rho = p->sun_rho / (r * r);
+        double ux = p->sun_v * x / r, uy = p->sun_v * y / r;
+        double pr = p->sun_p / std::pow(r, two * E.gamma);
+        double e0 = pr / hgamma + half * rho * p->sun_v * p->sun_v;
+        U(i, j, k, URHO) = rho;
+        U(i, j, k, UMOMX) = rho * ux; U(i, j, k, UMOMY) = rho * uy; U(i, j, k, UMOMZ) = zero;
+        U(i, j, k, UENG) = e0;
+        U(i, j, k, UBX) = zero; U(i, j, k, UBY) = zero; U(i, j, k, UBZ) = zero;
+        if (nf > 1) {
+          U(i, j, k, iRhoN + URHO) = p->net_rho;
+          U(i, j, k, iRhoN + UMOMX) = p->net_rho * one;
+          U(i, j, k, iRhoN + UMOMY) = zero;
+          U(i, j, k, iRhoN + UMOMZ) = zero;
+          U(i, j, k, iRhoN + UENG) = p->net_p / hgamma + half * p->net_rho;
+        }
+      }
+      if (nf > 2) {
+        int sh = iRhoN + WNUM_E;
+        U(i, j, k, sh + URHO) = netFluid2Dens;
+        U(i, j, k, sh + UMOMX) = zero; U(i, j, k, sh + UMOMY) = zero; U(i, j, k, sh + UMOMZ) = zero;
+        U(i, j, k, sh + UENG) = netFluid2Pres / hgamma;
+      }
+      if (nf > 3) {
+        int sh = iRhoN + 2 * WNUM_E;
+        U(i, j, k, sh + URHO) = netFluid3Dens;
+        U(i, j, k, sh + UMOMX) = zero; U(i, j, k, sh + UMOMY) = zero; U(i, j, k, sh + UMOMZ) = zero;
+        U(i, j, k, sh + UENG) = netFluid3Pres / hgamma;
+      }
+    }
+    return MFM_OK;
+  }
   if (p->problem == MFM_PROBLEM_RIEMANN) {
     const double* L = p->rp_left;
     const double* R = p->rp_right;
@@ -2792,6 +3157,13 @@ int oracle_region(const mfm_params* p, const double* w, int i, int j, int k,
 double oracle_max_wave_speed(const mfm_params* p, const double* u) {
   Eq E(p, 1.0, 1.0);
   return max_wave_speed_cell(E, u);
+}
+
+// The geometric sources of coord_sys = 1 of one cell in row j.
+void oracle_axisym_source(const mfm_params* p, const double* w, int j, double dt,
+                          double dx, double* s) {
+  Eq E(p, 1.0, 1.0);
+  axisym_source_cell(E, w, j, dt, dx, s);
 }
 
 void oracle_floors(const mfm_params* p, double* u) {

@@ -66,6 +66,7 @@ def load():
     lib.oracle_max_wave_speed.argtypes = [P, dp]
     lib.oracle_max_wave_speed.restype = C.c_double
     lib.oracle_floors.argtypes = [P, dp]
+    lib.oracle_axisym_source.argtypes = [P, dp, C.c_int, C.c_double, C.c_double, dp]
     BP = C.POINTER(mfmhd.Box)
     F = C.POINTER(mfmhd.Fab)
     lib.oracle_level_exchange.argtypes = [P, BP, C.c_int, F, C.c_int]
@@ -261,6 +262,14 @@ def region(p, w, i, j, k, dx):
 
 def max_wave_speed(p, u):
     return load().oracle_max_wave_speed(C.byref(p), _ptr(_vec(u)))
+
+
+def axisym_source(p, w, j, dt, dx):
+    """SOURCEAXISYMMETRIC[_MF] of one cell in row j (the conserved-variable
+    increments of one step)."""
+    s = np.zeros(32)
+    load().oracle_axisym_source(C.byref(p), _ptr(_vec(w)), j, dt, dx, _ptr(s))
+    return s[:len(w)]
 
 
 def floors(p, u):

@@ -58,7 +58,8 @@ def test_fill_periodic_refuses_bad_boxes():
     ctx.close()
 
 
-def test_kernel_timing_counts_launches_and_keeps_results(oracle_mod):
+@pytest.mark.parametrize("fused", [True, False])
+def test_kernel_timing_counts_launches_and_keeps_results(oracle_mod, fused):
     n = 16
     p, dx = helio_setup(oracle_mod, n, 4, mfmhd.PREC_FP32_FLUX)
     nc = mfmhd.num_components(p)
@@ -68,16 +69,22 @@ def test_kernel_timing_counts_launches_and_keeps_results(oracle_mod):
     U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
     dt = 0.5 * oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)[0]
     ctx = mfmhd.Context(p)
+    ctx.set_stage_kernel(fused)
     plain = gpu_update(ctx, U, lo, vlo, vhi, dt, dx)
     ctx.set_timing(True)
     timed = gpu_update(ctx, U, lo, vlo, vhi, dt, dx)
     assert np.array_equal(plain["U"], timed["U"]) and plain["dt"] == timed["dt"]
     # one RK2 step: one update per stage, the corrector's over the valid box
+    # (the fused stage kernel, or the separate flux kernels and k_update)
+    upd, other = (mfmhd.KT_STAGE, mfmhd.KT_UPDATE) if fused else (mfmhd.KT_UPDATE, mfmhd.KT_STAGE)
     for st in (0, 1):
-        ms, launches, items = ctx.timing(st, mfmhd.KT_UPDATE)
+        ms, launches, items = ctx.timing(st, upd)
         assert launches == 1 and ms > 0.0, (st, ms, launches)
         assert items >= n ** 3
-    assert ctx.timing(1, mfmhd.KT_UPDATE)[2] == n ** 3
+        assert ctx.timing(st, other)[1] == 0
+        assert ctx.timing(st, mfmhd.KT_FLUX_PLASMA)[1] == (0 if fused else 3)
+        assert ctx.timing(st, mfmhd.KT_FLUX_BC)[1] == 3   # boundary faces of x, y, z
+    assert ctx.timing(1, upd)[2] == n ** 3
     assert ctx.timing(0, mfmhd.KT_PRIM)[1] == 1
     assert ctx.timing(0, mfmhd.KT_PREDICT)[1] == 0    # no Hancock predictor in RK2
     # switching timing on again starts new sums
