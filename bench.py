@@ -46,11 +46,17 @@ def finite_dt(dtn, where):
         raise RuntimeError("%s: state went non-finite (next dt %r)" % (where, dtn))
     return dtn
 
-# HBM bytes per k_update launch (average over the two RK2 stages) measured
-# with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the default workload
-# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0 from a 1 GiB copy; summary in
-# profiles/k_update_helio256_traffic.md).  Valid for that workload only.
-TRAFFIC = {(256, "fp32flux"): 14241000000.0}
+def note(msg):
+    """One progress line on stderr (the JSON result stays alone on stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+# HBM bytes per launch of the dominant kernel (average over the two RK2
+# stages), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
+# default workload and scaled by what the counters read for the library's
+# own copy kernel over 1 GiB (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); summary
+# in profiles/k_update_helio256_traffic.md.  Valid for that workload only.
+TRAFFIC = {(256, "fp32flux", "k_update"): 1.4068e10}
 
 
 def c3_params(n, prec):
@@ -67,15 +73,30 @@ def c3_params(n, prec):
     return p, dx
 
 
-def cpu_baseline(n_sample, threads, budget_s=20.0):
-    """The CPU oracle (FP64 restatement of the reference's path) on a
-    bounded sample of the same workload: a n_sample^3 patch of the
-    heliospheric state around the sun, OpenMP over z planes."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle
-    n = 256
-    p, dx = c3_params(n, mfmhd.PREC_FP64)
-    ng = 4
+def host_cpu():
+    """nproc, the CPU model and the SMT state of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    smt = "unknown"
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as fh:
+            smt = "on" if fh.read().strip() == "1" else "off"
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model, "smt": smt}
+
+
+def _oracle_rate(oracle, p, dx, n_sample, threads, warmup=3, timed=10):
+    """Median over `timed` steps (after `warmup`) of the oracle's RK2 update
+    of an n_sample^3 patch of the 256^3 heliospheric state around the sun."""
+    n, ng = 256, 4
     c0 = n // 2 - n_sample // 2
     vlo = (c0, c0, c0)
     vhi = (c0 + n_sample - 1,) * 3
@@ -84,59 +105,77 @@ def cpu_baseline(n_sample, threads, budget_s=20.0):
     shape = (23, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
     U = oracle.init_problem(p, shape, lo, lo, hi, dx)
     dt0, _ = oracle.compute_dt(p, U, lo, vlo, vhi, dx)
-    dt = 0.5 * dt0
     times = []
-    t_start = time.time()
-    while True:
+    for s in range(warmup + timed):
         t0 = time.perf_counter()
-        r = oracle.patch_update(p, U, lo, vlo, vhi, dt, dx, nthreads=threads)
-        times.append(time.perf_counter() - t0)
+        r = oracle.patch_update(p, U, lo, vlo, vhi, 0.5 * dt0, dx, nthreads=threads)
+        if s >= warmup:
+            times.append(time.perf_counter() - t0)
         assert r["rc"] == 0
-        if time.time() - t_start > budget_s or len(times) >= 6:
-            break
-    med = float(np.median(times[1:] if len(times) > 2 else times))
-    cells = n_sample ** 3
-    return {"value": cells / med, "unit": "cell-updates/s", "cores": threads,
+    return n_sample ** 3 / float(np.median(times)), len(times)
+
+
+def cpu_baseline(n_sample, threads, n_single=32):
+    """The CPU oracle (the FP64 C++ restatement of the reference's path,
+    -O3 -march=x86-64-v3 -ffp-contract=off -fno-fast-math, OpenMP over z
+    planes) on bounded samples of the same workload: an n_sample^3 patch on
+    `threads` cores and an n_single^3 patch on 1 core; 3 warm-up steps, then
+    the median of 10."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    p, dx = c3_params(256, mfmhd.PREC_FP64)
+    v_all, k = _oracle_rate(oracle, p, dx, n_sample, threads)
+    v_one, _ = _oracle_rate(oracle, p, dx, n_single, 1)
+    return {"value": v_all, "unit": "cell-updates/s", "cores": threads,
             "kind": "port",
-            "sample": "oracle FP64 RK2 update of a %d^3 patch of the 256^3 "
-                      "4-fluid heliospheric state (+4 ghosts), median of %d "
-                      "steps, OpenMP %d threads" % (n_sample, len(times), threads)}
+            "sample": "oracle FP64 RK2 update of a %d^3 patch of the 256^3 4-fluid "
+                      "heliospheric state (+4 ghosts) around the sun, 3 warm-up steps, "
+                      "median of %d, OpenMP %d threads" % (n_sample, k, threads),
+            "single_core": {"value": v_one, "cores": 1,
+                            "sample": "%d^3 patch, same state, 3 warm-up, median of 10"
+                                      % n_single},
+            "host": host_cpu()}
 
 
-def serve_bench(prec, n_req=48, steps=20, rate_hz=400.0, seed=7):
-    """Ensemble serving: independent 3-D 4-fluid heliospheric solves with
-    grids n^3, n even in [16, 64], solar-wind speed ~ U[350, 800] km/s and LISM
-    field ~ U[2, 5] uG, arriving as a Poisson process, continuously batched
-    by mfm_server_* under an HBM budget.  Reports the p50/p99 per-request
-    step latency (one server iteration advances every active request one
-    RK2 step) beside the throughput."""
+def serve_bench(prec, n_req=512, steps=20, rate_hz=4096.0, seed=7):
+    """Ensemble serving of the 2-D axisymmetric SW/LISM problem (the
+    reference's swlism2F deck: 1500 AU domain, sun at 600 AU on the axis,
+    plasma + 1 neutral = 13 components, first order in time, cfl 0.25,
+    initial_cfl 0.1, max_dt_growth 1.02): n_req independent requests with
+    grids nx x ny, nx and ny ~ U[64, 512], solar-wind speed ~ U[350, 800] km/s and
+    LISM field ~ U[2, 5] uG, arriving as a Poisson process of rate rate_hz,
+    continuously batched by mfm_server_* under a 64 GiB HBM budget.  Fluxes:
+    8-wave Roe on Roe-averaged states (the deck's arithmetic-mean Roe loses
+    positivity at the wind / LISM interface of SWLISMINIT's state within a
+    few steps).  Reports the p50/p99 per-request step latency (one server
+    iteration advances every active request one step) beside the
+    throughput."""
     rng = np.random.default_rng(seed)
     reqs = []
     for _ in range(n_req):
-        n = 2 * int(rng.integers(8, 33))  # even: the sun sits on a node
+        nx, ny = (int(v) for v in rng.integers(64, 513, size=2))
         sunV = float(rng.uniform(3.5e7, 8.0e7))
         lismB = float(rng.uniform(2e-6, 5e-6))
-        dx = 1.0 / n
-        p = mfmhd.helio_params(lismN=0.06, lismV=2.6e6, lismT=6527.0, lismB=lismB,
-                               lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0),
-                               lismUdir=(-1.0, 0.0, 0.0), sunN=SUN_N_1AU, sunV=sunV,
-                               sunXYZ=(0.5, 0.5, 0.5), R0=0.15, initR=0.3,
-                               netN=0.1, dx=dx, num_cells=(n, n, n))
-        p.nfluids = 4
-        p.time_approx = mfmhd.TIME_RK2
-        p.flux = mfmhd.FLUX_ROE8
+        dx = 1500.0 / nx
+        p = mfmhd.swlism_params(sunV=sunV, lismB=lismB, sunXY=(600.0, 0.0),
+                                R0=max(10.0, 2.6 * dx), initR=max(80.0, 8.0 * dx),
+                                dx=dx, num_cells=(nx, ny), nfluids=2)
+        p.time_approx = mfmhd.TIME_FIRST_ORDER
+        p.flux = mfmhd.FLUX_ROE8_ROEAVG
         p.precision = prec
-        reqs.append((p, dx, n))
+        reqs.append((p, dx, nx * ny))
     arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, size=n_req))
     srv = mfmhd.Server(reqs[0][0], 64.0 * (1 << 30))
     t0 = time.perf_counter()
     nxt, ids, meta = 0, [], {}
+    cells = 0.0
     while True:
         now = time.perf_counter() - t0
         while nxt < n_req and arrivals[nxt] <= now:
-            p, dx, n = reqs[nxt]
-            ids.append(srv.submit(p, dx, steps, CFL, CFL0, DT_GROWTH))
-            meta[ids[-1]] = (n, p.sun_v, p.lism_bx)
+            p, dx, ncell = reqs[nxt]
+            ids.append(srv.submit(p, dx, steps, 0.25, 0.1, 1.02))
+            meta[ids[-1]] = (ncell, p.sun_v, p.lism_bx)
+            cells += float(ncell) * steps
             nxt += 1
         na, npd, _ = srv.step()
         for r in list(ids):
@@ -154,12 +193,15 @@ def serve_bench(prec, n_req=48, steps=20, rate_hz=400.0, seed=7):
     st = srv.stats()
     srv.close()
     return {"requests": n_req, "steps_per_request": steps,
-            "grid": "n^3, n even in [16, 64], 4 fluids (23 comps), RK2",
+            "workload": "SW/LISM 2-D axisymmetric, nx x ny, nx, ny ~ U[64, 512], sunV ~ "
+                        "U[350, 800] km/s, lismB ~ U[2, 5] uG, 2 fluids (13 comps), "
+                        "first order, Roe8 on Roe averages, minmod",
             "arrival": "Poisson %.0f /s" % rate_hz,
             "p50_step_ms": st["p50_ms"], "p99_step_ms": st["p99_ms"],
             "iterations": st["iterations"],
+            "cell_updates": cells,
             "cell_updates_per_s": st["cell_updates"] / (st["busy_ms"] * 1e-3),
-            "wall_s": wall}
+            "wall_s": wall, "wall_cell_updates_per_s": cells / wall}
 
 
 def amr_bench(prec, n_patches=3000, levels=3, steps=5, warmup=1, seed=11):
@@ -434,6 +476,118 @@ def hancock_bench(prec, n=256, steps=10, warmup=3, walk=None):
             "kernels": kern}
 
 
+def brio_wu_bench(n=800, t_end=0.1, cfl=0.8):
+    """The 1-D Brio-Wu shock tube at its stated size: n cells on [0, 1],
+    gamma = 2, Roe 8-wave on arithmetic means, minmod, 8-wave sources, RK2,
+    CFL 0.8, fixed-state boundaries, FP64, run to t = 0.1 (a 2-D patch one
+    cell high, periodic in y).  Too small to fill the GPU: reported for
+    latency (launch-bound), not for the roofline."""
+    S4PI = math.sqrt(4.0 * math.pi)
+    L = [1.0, 0, 0, 0, 1.0, 0.75 * S4PI, S4PI, 0]
+    R = [0.125, 0, 0, 0, 0.1, 0.75 * S4PI, -S4PI, 0]
+    p = mfmhd.riemann_params(2.0, L, R, 0.5)
+    p.spacedim = 2
+    p.time_approx = mfmhd.TIME_RK2
+    p.flux = mfmhd.FLUX_ROE8
+    p.precision = mfmhd.PREC_FP64
+    for v in range(8):
+        p.limiter[v] = mfmhd.LIM_MINMOD
+    p.domain_lo[:] = [0, 0, 0]
+    p.domain_hi[:] = [n - 1, 0, 0]
+    p.periodic[:] = [0, 1, 0]
+    dx = 1.0 / n
+    ctx = mfmhd.Context(p)
+    ng = ctx.nghost
+    lo, hi = (-ng, -ng, 0), (n - 1 + ng, ng, 0)
+    U = mfmhd.DeviceArray((8, 1, 1 + 2 * ng, n + 2 * ng))
+    Uf = mfmhd.fab(U.ptr, lo, hi, 8)
+    valid = mfmhd.Box.of((0, 0, 0), (n - 1, 0, 0))
+    ctx.init_problem(Uf, mfmhd.Box.of(lo, hi), dx)
+    ctx.fill_periodic(Uf, valid)
+    dtn, _ = ctx.compute_dt(Uf, valid, dx)
+    t, steps = 0.0, 0
+    mfmhd.synchronize()
+    e0, e1 = mfmhd.Event(), mfmhd.Event()
+    e0.record()
+    while t < t_end * (1 - 1e-12):
+        dt = min(cfl * dtn, t_end - t)
+        ctx.fill_periodic(Uf, valid)
+        dtn = finite_dt(ctx.update_state(Uf, valid, dt, dx, t)[0], "Brio-Wu")
+        t += dt
+        steps += 1
+    e1.record()
+    mfmhd.synchronize()
+    ms = e0.elapsed_ms(e1)
+    ctx.close()
+    U.free()
+    return {"workload": "Brio-Wu 1-D, %d cells, Roe8, minmod, RK2, CFL %.1f, to t=%.2g, FP64"
+                        % (n, cfl, t_end),
+            "steps": steps, "t": t, "total_ms": ms, "ms_per_step": ms / steps,
+            "cell_updates_per_s": n * steps / (ms * 1e-3)}
+
+
+def orszag_glm_bench(n=512, steps=1000, cfl=0.5):
+    """The 2-D Orszag-Tang vortex at its stated size: n^2 periodic on
+    [0, 1]^2, gamma = 5/3, GLM (Dedner-Roe: flux 1 with divB_cleaning 5,
+    8-wave off, Ch = dx / dt_raw and Cp = sqrt(Ch dx) each step as the level
+    driver sets them, dt_raw = dx / c_max), minmod, RK2, FP64, CFL 0.5 (the
+    reference's Orszag deck; 0.8 is unstable for this unsplit 2-D update),
+    `steps` steps with the periodic ghost fill in each."""
+    p = mfmhd.orszag_params(5.0 / 3.0)
+    p.spacedim = 2
+    p.nfluids = 1
+    p.dedner = 1
+    p.eight_wave = 0
+    p.flux = mfmhd.FLUX_ROE8
+    p.time_approx = mfmhd.TIME_RK2
+    p.precision = mfmhd.PREC_FP64
+    for v in range(8):
+        p.limiter[v] = mfmhd.LIM_MINMOD
+    p.domain_lo[:] = [0, 0, 0]
+    p.domain_hi[:] = [n - 1, n - 1, 0]
+    p.periodic[:] = [1, 1, 0]
+    dx = 1.0 / n
+    ctx = mfmhd.Context(p)
+    ng = ctx.nghost
+    nc = ctx.ncomp
+    lo, hi = (-ng, -ng, 0), (n - 1 + ng, n - 1 + ng, 0)
+    U = mfmhd.DeviceArray((nc, 1, n + 2 * ng, n + 2 * ng))
+    Uf = mfmhd.fab(U.ptr, lo, hi, nc)
+    valid = mfmhd.Box.of((0, 0, 0), (n - 1, n - 1, 0))
+    ctx.init_problem(Uf, mfmhd.Box.of(lo, hi), dx)
+    ctx.fill_periodic(Uf, valid)
+    dtn, _ = ctx.compute_dt(Uf, valid, dx)
+    t = 0.0
+
+    def step():
+        nonlocal t, dtn
+        dt = cfl * dtn
+        ch = dx / dtn
+        ctx.set_dedner(ch, math.sqrt(ch * dx))
+        ctx.fill_periodic(Uf, valid)
+        dtn = finite_dt(ctx.update_state(Uf, valid, dt, dx, t)[0], "Orszag-Tang")
+        t += dt
+
+    for _ in range(5):
+        step()
+    mfmhd.synchronize()
+    e0, e1 = mfmhd.Event(), mfmhd.Event()
+    e0.record()
+    for _ in range(steps):
+        step()
+    e1.record()
+    mfmhd.synchronize()
+    ms = e0.elapsed_ms(e1) / steps
+    ctx.close()
+    U.free()
+    rate = n * n / (ms * 1e-3)
+    return {"workload": "Orszag-Tang 2-D %d^2 periodic, GLM (Dedner-Roe), minmod, RK2, "
+                        "CFL %.1f, FP64" % (n, cfl),
+            "steps": steps, "t_end": t, "ms_per_step": ms, "cell_updates_per_s": rate,
+            "roofline_frac": rate * 5 * nc * 8 / HBM_PEAK,
+            "alg_bytes_per_cell_update": 5 * nc * 8}
+
+
 def orszag_bench(nxy=512, nz=32, steps=10, warmup=3):
     """The scheme of the reference's Orszag-Tang deck (main/inputs/
     orszag.inputs: Hancock, characteristic slopes with check_tvd = 1, van
@@ -610,10 +764,10 @@ def main():
     ap.add_argument("--precision", choices=["fp32flux", "fp64"], default="fp32flux")
     ap.add_argument("--cfl", type=float, default=CFL)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=40)
+    ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the ensemble-serving measurement")
-    ap.add_argument("--serve-requests", type=int, default=48)
+    ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default work walk instead of autotuning it")
     ap.add_argument("--no-amr", action="store_true",
@@ -626,6 +780,9 @@ def main():
                     help="skip the Orszag-Tang deck scheme (characteristic slopes) timing")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-buffer (PCIe-inclusive) update measurement")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the Brio-Wu (800 cells to t=0.1) and Orszag-Tang GLM "
+                         "(512^2, 1000 steps) runs")
     ap.add_argument("--no-fill", action="store_true",
                     help="skip the level ghost-fill measurement")
     ap.add_argument("--fused-stage", action="store_true",
@@ -634,8 +791,8 @@ def main():
     ap.add_argument("--tz", type=int, default=16,
                     help="planes per z-chunk of the fused stage kernel")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
-                    help="after the run, copy MIB MiB device-to-device once "
-                         "(calibrates HBM byte counters under rocprofv3 --pmc)")
+                    help="after the run, launch the library's copy kernel over MIB MiB "
+                         "twice (calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path "
@@ -746,7 +903,10 @@ def main():
                                 "alg_bytes_per_launch": tot_bytes / launches,
                                 "avg_launch_ms": tot_ms / launches,
                                 "achieved": k_achieved / 1e9,
-                                "frac": k_achieved / HBM_PEAK}},
+                                "frac": k_achieved / HBM_PEAK,
+                                "traffic_over_alg": (
+                                    TRAFFIC[(n, args.precision, kname)] / (tot_bytes / launches)
+                                    if (n, args.precision, kname) in TRAFFIC else None)}},
         "step_hbm_frac": step_achieved / HBM_PEAK,
         "kernels": kern,
         "wall_s_timed": wall,
@@ -760,28 +920,40 @@ def main():
     U.free()
     if final32 is not None:
         out["fp64"] = fp64_bench(n, dts, args.warmup, walk, final32)
+        note("FP64 run done")
         del final32
     if args.calib_copy > 0:
-        nb = args.calib_copy << 20
-        a = mfmhd.DeviceArray((nb // 8,), zero=True)
-        b = mfmhd.DeviceArray((nb // 8,))
-        b.copy_from(a)
-        mfmhd.synchronize()
-        a.free()
-        b.free()
+        # the library's copy kernel (k_copy: reads and writes MIB MiB per
+        # launch, one warm-up launch and one timed): the reference dispatch
+        # for scaling FETCH_SIZE / WRITE_SIZE to bytes
+        mfmhd.hbm_ceiling(mfmhd.CEIL_COPY, args.calib_copy << 20, 1)
+    note("timed step done: %.3f ms/step" % ms)
     if not args.no_serve:
         out["ensemble"] = serve_bench(prec, n_req=args.serve_requests)
+        note("ensemble done")
     if not args.no_amr:
         out["amr_batched"] = amr_bench(prec)
+        note("batched AMR done")
     if not args.no_hancock:
         out["hancock"] = hancock_bench(prec, n, args.steps, args.warmup, walk)
+        note("Hancock done")
     if not args.no_fill:
         out["ghost_fill"] = fill_bench()
     if not args.no_orszag:
         out["orszag_char"] = orszag_bench()
+    if not args.no_small:
+        out["brio_wu_800"] = brio_wu_bench()
+        out["orszag_glm_512"] = orszag_glm_bench()
+        note("Brio-Wu and Orszag-Tang done")
     out["roofline"]["copy_ceiling_GBps"] = copy_ceiling() / 1e9
+    # the library's own streaming kernels over 2 GiB arrays
+    out["roofline"]["ceilings_GBps"] = {
+        name: mfmhd.hbm_ceiling(kind, 2 << 30)[0]
+        for name, kind in (("read", mfmhd.CEIL_READ), ("copy", mfmhd.CEIL_COPY),
+                           ("triad", mfmhd.CEIL_TRIAD))}
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
+        note("CPU baseline (oracle, %d threads and 1) ..." % threads)
         cb = cpu_baseline(args.cpu_sample, threads)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = value / cb["value"]

@@ -248,6 +248,16 @@ int mfm_ctx_get_timing(mfm_ctx* ctx, int stage, int kind, double* ms,
  * scheme and characteristic slopes always use the separate kernels.       */
 int mfm_ctx_set_stage_kernel(mfm_ctx* ctx, int fused, int tz);
 
+/* HBM ceilings of the current device, measured with the library's own
+ * streaming kernels (no counterpart in the reference, a CPU code): read
+ * (sum of an array), copy (b = a) and triad (a = b + s c) over arrays of
+ * `bytes` each (a multiple of 16, >= 1 MiB), 16-byte accesses, 8 blocks of
+ * 256 threads per CU.  One warm-up launch, then the best of `reps`;
+ * gbps = bytes moved (1x, 2x or 3x `bytes`) / time.  Allocates its arrays
+ * with hipMalloc and frees them before returning.                          */
+enum { MFM_CEIL_READ = 0, MFM_CEIL_COPY = 1, MFM_CEIL_TRIAD = 2 };
+int mfm_hbm_ceiling(int kind, size_t bytes, int reps, double* gbps, double* best_ms);
+
 /* ---- one patch: PatchMHDAM::updateState ---------------------------------
  * U      : device FAB, box = valid grown by >= numGhostCells, ghosts filled
  *          (exchange / interpolation) as the reference requires; updated in

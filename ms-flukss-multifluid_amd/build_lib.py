@@ -43,6 +43,7 @@ UNITS = [
     ("mfm_fill_plan.cpp", ["-ffp-contract=off"]),
     ("mfm_chombo.cpp", []),
     ("mfm_noalloc.hip", ["-ffp-contract=off"]),
+    ("mfm_ceiling.hip", []),
 ]
 
 

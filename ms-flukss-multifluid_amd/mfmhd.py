@@ -100,7 +100,7 @@ EXPORTS = [
     "mfm_event_destroy", "mfm_event_record", "mfm_event_elapsed_ms",
     "mfm_ctx_set_timing", "mfm_ctx_get_timing", "mfm_ctx_set_graphs",
     "mfm_ctx_set_walk", "mfm_ctx_get_walk", "mfm_ctx_autotune",
-    "mfm_ctx_set_stage_kernel",
+    "mfm_ctx_set_stage_kernel", "mfm_hbm_ceiling",
     "mfm_server_create", "mfm_server_destroy", "mfm_server_last_error",
     "mfm_server_submit", "mfm_server_step", "mfm_server_request_info",
     "mfm_server_fetch", "mfm_server_release", "mfm_server_stats",
@@ -158,6 +158,7 @@ def load(path=LIB_PATH):
         C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
         C.c_double, C.c_double, ip]
     lib.mfm_num_components.argtypes = [P]
+    lib.mfm_hbm_ceiling.argtypes = [C.c_int, C.c_size_t, C.c_int, dp, dp]
     lib.mfm_num_ghost_cells.argtypes = [P]
     lib.mfm_ctx_create.argtypes = [P, C.POINTER(C.c_void_p)]
     lib.mfm_ctx_destroy.argtypes = [C.c_void_p]
@@ -651,6 +652,19 @@ def fill_time_coef(time, t_coarse_new, dt_coarse, dt_fine, ref_ratio):
 # ---- device memory and timing on the library's runtime ---------------------
 def device_count():
     return load().mfm_dev_count()
+
+
+CEIL_READ, CEIL_COPY, CEIL_TRIAD = range(3)
+
+
+def hbm_ceiling(kind, nbytes=1 << 30, reps=5):
+    """(GB/s, best ms) of the library's read / copy / triad kernel over
+    arrays of nbytes each (bytes moved: 1x, 2x, 3x nbytes)."""
+    g, ms = C.c_double(), C.c_double()
+    rc = load().mfm_hbm_ceiling(int(kind), int(nbytes), int(reps), C.byref(g), C.byref(ms))
+    if rc != OK:
+        raise MfmError(rc, "mfm_hbm_ceiling failed")
+    return g.value, ms.value
 
 
 def synchronize():

@@ -1,5 +1,6 @@
 """Ensemble serving (mfm_server_*): continuous batching of independent
-heliospheric solves admitted by an HBM budget.
+heliospheric (3-D) and SW/LISM (2-D axisymmetric) solves admitted by an HBM
+budget.
 
 A request's result must not depend on what else is in flight: the per-cell
 arithmetic of a patch is the same in any batch, so the FAB of a request run
@@ -141,4 +142,85 @@ def test_hbm_budget_limits_admission():
             break
     assert not ids
     assert peak == 2 and waited
+    srv.close()
+
+
+def swlism_req(n, sunV=4.5e7, lismB=1.5e-6, nf=2, prec=mfmhd.PREC_FP64):
+    """A SW/LISM request as the ensemble's: an n x n grid over the deck's
+    1500 AU, the sun at 600 AU on the axis, first order in time, Roe-averaged
+    8-wave fluxes."""
+    dx = 1500.0 / n
+    p = mfmhd.swlism_params(sunV=sunV, lismB=lismB, sunXY=(600.0, 0.0), R0=max(10.0, 2.6 * dx),
+                            initR=max(80.0, 8.0 * dx), dx=dx, num_cells=(n, n), nfluids=nf)
+    p.time_approx = mfmhd.TIME_FIRST_ORDER
+    p.flux = mfmhd.FLUX_ROE8_ROEAVG
+    p.precision = prec
+    return p, dx
+
+
+def test_swlism_served_request_matches_oracle(oracle_mod):
+    """The deck's step control (initial_cfl 0.1, cfl 0.25, max_dt_growth
+    1.02) on a served SW/LISM request against the oracle stepped alike."""
+    n = 40
+    p, dx = swlism_req(n)
+    srv = mfmhd.Server(p, 8 << 30)
+    rid = srv.submit(p, dx, 4, 0.25, 0.1, 1.02)
+    run_until_done(srv, lambda: [rid])
+    Ug = srv.fetch(rid)
+    inf = srv.info(rid)
+    srv.close()
+    lo, hi = inf["lo"], inf["hi"]
+    vlo, vhi = (0, 0, 0), (n - 1, n - 1, 0)
+    shape = (mfmhd.num_components(p), 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+    U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
+    raw, _ = oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)
+    t, dt = 0.0, 0.1 * raw
+    for _ in range(4):
+        r = oracle_mod.patch_update(p, U, lo, vlo, vhi, dt, dx, time=t)
+        assert r["rc"] == 0
+        U = r["U"]
+        t += dt
+        dt = min(0.25 * r["dt"], 1.02 * dt)
+    assert abs(inf["t"] - t) <= 1e-12 * t
+    err = rel_err_per_comp(valid_view(Ug, lo, vlo, vhi), valid_view(U, lo, vlo, vhi)).max()
+    assert err <= TOL64, err
+
+
+def test_swlism_request_alone_equals_request_in_mix():
+    pa, dxa = swlism_req(48, sunV=6.0e7, lismB=2.5e-6)
+    s1 = mfmhd.Server(pa, 8 << 30)
+    a = s1.submit(pa, dxa, 5, 0.25, 0.1, 1.02)
+    run_until_done(s1, lambda: [a])
+    Ua = s1.fetch(a)
+    s1.close()
+    s2 = mfmhd.Server(pa, 8 << 30)
+    others = [swlism_req(64, sunV=3.5e7, lismB=5e-6), swlism_req(72, sunV=8.0e7, lismB=2e-6)]
+    ids = [s2.submit(p, dx, 7, 0.25, 0.1, 1.02) for p, dx in others]
+    late = {}
+
+    def arrivals(it):
+        if it == 1:
+            late["a"] = s2.submit(pa, dxa, 5, 0.25, 0.1, 1.02)
+            ids.append(late["a"])
+
+    run_until_done(s2, lambda: list(ids), hook=arrivals)
+    for r in ids:
+        assert s2.info(r)["state"] == mfmhd.REQ_DONE, s2.info(r)
+    Um = s2.fetch(late["a"])
+    s2.close()
+    assert np.array_equal(Ua, Um)
+
+
+def test_server_refuses_another_class():
+    """A 2-D axisymmetric request is not of a 3-D heliospheric server's
+    class (nor the other way round)."""
+    ph, dxh = req_params(16)
+    ps, dxs = swlism_req(40)
+    srv = mfmhd.Server(ph, 8 << 30)
+    with pytest.raises(mfmhd.MfmError):
+        srv.submit(ps, dxs, 1, 0.25)
+    srv.close()
+    srv = mfmhd.Server(ps, 8 << 30)
+    with pytest.raises(mfmhd.MfmError):
+        srv.submit(ph, dxh, 1, 0.5)
     srv.close()
