@@ -204,23 +204,26 @@ def serve_bench(prec, n_req=512, steps=20, rate_hz=4096.0, seed=7):
             "wall_s": wall, "wall_cell_updates_per_s": cells / wall}
 
 
-def amr_bench(prec, n_patches=3000, levels=3, steps=5, warmup=1, seed=11):
-    """Batched AMR patch tables: `levels` levels (ratio 2), about n_patches
-    patches in all with edges drawn from {8, 16, 32} (seeded), 4 ghost cells,
-    4 fluids, on the heliospheric state of each level's grid.  Each level is
-    one mfm_level_update_dev call (every kernel launched once for all its
-    patches); a step updates every level once.  Returns cell-updates/s over
-    all levels."""
+def amr_hierarchy(prec, n_patches=3000, levels=3, seed=11, n_blobs=12):
+    """The batched-AMR workload: `levels` nested levels (ratio 2) of the
+    heliospheric problem, about n_patches patches in all (n_patches/levels
+    per level) with edges drawn uniformly from {8, 16, 32}, each in its own
+    32^3 slot of a cube of m^3 slots; level L+1 refines the central half of
+    level L (domain 32 m 2^L cells per side), so the three levels cover
+    nested regions around the sun.  Seeded Gaussian blobs of density and
+    pressure (n_blobs, amplitude U[0.2, 1], width U[0.02, 0.08] of the
+    domain) sit on the HELIOINIT background.  Returns (levels, blobs)."""
     rng = np.random.default_rng(seed)
     per_level = n_patches // levels
-    ng = 4
-    nc = 23
+    m = int(np.ceil(per_level ** (1.0 / 3.0)))
+    side = 32 * m
+    blobs = [(tuple(rng.uniform(0.3, 0.7, size=3)), float(rng.uniform(0.2, 1.0)),
+              float(rng.uniform(0.2, 1.0)), float(rng.uniform(0.02, 0.08)))
+             for _ in range(n_blobs)]
     lv = []
-    total_cells = 0
     for L in range(levels):
-        # patches sit in 32^3 slots of a cube of side m slots around the sun
-        m = int(np.ceil(per_level ** (1.0 / 3.0)))
-        n_dom = 32 * m
+        n_dom = side * 2 ** L
+        off = (n_dom - side) // 2
         dx = 1.0 / n_dom
         p = mfmhd.helio_params(lismN=0.06, lismV=2.6e6, lismT=6527.0, lismB=3e-6,
                                lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0),
@@ -233,31 +236,93 @@ def amr_bench(prec, n_patches=3000, levels=3, steps=5, warmup=1, seed=11):
         p.precision = prec
         slots = rng.permutation(m ** 3)[:per_level]
         edges = rng.choice([8, 16, 32], size=per_level)
-        boxes, offs, off = [], [], 0
+        boxes = []
         for q in range(per_level):
             sx, sy, sz = slots[q] % m, (slots[q] // m) % m, slots[q] // (m * m)
             e = int(edges[q])
-            vlo = (32 * sx, 32 * sy, 32 * sz)
-            vhi = tuple(v + e - 1 for v in vlo)
-            boxes.append((vlo, vhi))
+            vlo = (off + 32 * sx, off + 32 * sy, off + 32 * sz)
+            boxes.append((vlo, tuple(v + e - 1 for v in vlo)))
+        lv.append({"p": p, "dx": dx, "boxes": boxes, "n_dom": n_dom})
+    return lv, blobs
+
+
+def apply_blobs(U, lo, dx, blobs, gamma):
+    """Multiply the plasma density and pressure of the conserved FAB U
+    (nc, nz, ny, nx; low corner lo) by 1 + sum of Gaussian blobs, keeping
+    the velocity and the field."""
+    nc, nz, ny, nx = U.shape
+    z, y, x = np.meshgrid((np.arange(nz) + lo[2] + 0.5) * dx, (np.arange(ny) + lo[1] + 0.5) * dx,
+                          (np.arange(nx) + lo[0] + 0.5) * dx, indexing="ij")
+    fr = np.ones_like(x)
+    fp = np.ones_like(x)
+    for (cx, cy, cz), ar, ap, w in blobs:
+        g = np.exp(-((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2) / (w * w))
+        fr += ar * g
+        fp += ap * g
+    rho = U[0]
+    ke = 0.5 * (U[1] ** 2 + U[2] ** 2 + U[3] ** 2) / rho
+    me = (U[5] ** 2 + U[6] ** 2 + U[7] ** 2) / (8.0 * math.pi)
+    pr = (gamma - 1.0) * (U[4] - ke - me)
+    U[0] = rho * fr
+    U[1:4] *= fr
+    U[4] = pr * fp / (gamma - 1.0) + ke * fr + me
+
+
+def amr_state(lv, blobs, ng=4, nc=23):
+    """Device buffers of the hierarchy: per level one DeviceArray holding
+    every patch's FAB (valid + ng ghosts) back to back, HELIOINIT on the
+    device, then the blobs.  Returns, per level, (ctx, buf, table, offsets)."""
+    out = []
+    for L in lv:
+        offs, off = [], 0
+        for vlo, vhi in L["boxes"]:
             offs.append(off)
-            off += nc * (e + 2 * ng) ** 3
-            total_cells += e ** 3
+            off += nc * int(np.prod([h - l + 1 + 2 * ng for l, h in zip(vlo, vhi)]))
         buf = mfmhd.DeviceArray((off,))
-        ctx = mfmhd.Context(p)
+        ctx = mfmhd.Context(L["p"])
         table = []
-        for (vlo, vhi), o in zip(boxes, offs):
+        for (vlo, vhi), o in zip(L["boxes"], offs):
             lo = tuple(v - ng for v in vlo)
             hi = tuple(v + ng for v in vhi)
             f = mfmhd.fab(buf.ptr + 8 * o, lo, hi, nc)
-            ctx.init_problem(f, mfmhd.Box.of(lo, hi), dx)
+            ctx.init_problem(f, mfmhd.Box.of(lo, hi), L["dx"])
             table.append((f, mfmhd.Box.of(vlo, vhi)))
-        lv.append({"ctx": ctx, "buf": buf, "table": table, "dx": dx, "dt": None,
-                   "dtn": None, "t": 0.0})
+        mfmhd.synchronize()
+        host = buf.numpy()
+        for (vlo, vhi), o in zip(L["boxes"], offs):
+            shp = (nc,) + tuple(h - l + 1 + 2 * ng for l, h in zip(vlo[::-1], vhi[::-1]))
+            U = host[o:o + int(np.prod(shp))].reshape(shp)
+            apply_blobs(U, tuple(v - ng for v in vlo), L["dx"], blobs, L["p"].gamma)
+        buf.upload(host)
+        del host
+        out.append((ctx, buf, table, offs))
     mfmhd.synchronize()
+    return out
+
+
+def amr_bench(prec, n_patches=3000, levels=3, steps=5, warmup=1, seed=11):
+    """The batched-AMR workload (amr_hierarchy: 3 nested levels, about 3000
+    patches of edge 8/16/32, Gaussian blobs on HELIOINIT), 4 ghost cells,
+    4 fluids.  Each level is one mfm_level_update_dev call (every kernel
+    launched once for all its patches) replayed from a captured hipGraph,
+    with the work walk autotuned on the level's largest patch; a step
+    updates every level once.  Returns cell-updates/s over all levels."""
+    lv, blobs = amr_hierarchy(prec, n_patches, levels, seed)
+    st = amr_state(lv, blobs)
+    total_cells = sum(int(np.prod([h - l + 1 for l, h in zip(vlo, vhi)]))
+                      for L in lv for vlo, vhi in L["boxes"])
+    state = []
+    for L, (ctx, buf, table, offs) in zip(lv, st):
+        ctx.set_graphs(True)
+        big = max(range(len(table)), key=lambda q: table[q][1].shape()[0])
+        f, b = table[big]
+        d0, _ = ctx.compute_dt(f, b, L["dx"])
+        ctx.autotune(f, b, 0.1 * d0, L["dx"])
+        state.append({"ctx": ctx, "buf": buf, "table": table, "dx": L["dx"], "dt": None,
+                      "dtn": None, "t": 0.0})
 
     def step():
-        for L in lv:
+        for L in state:
             if L["dtn"] is None:
                 L["dtn"] = min(L["ctx"].compute_dt(f, b, L["dx"])[0] for f, b in L["table"])
             dt = CFL0 * L["dtn"] if L["dt"] is None else min(CFL * L["dtn"], DT_GROWTH * L["dt"])
@@ -276,11 +341,14 @@ def amr_bench(prec, n_patches=3000, levels=3, steps=5, warmup=1, seed=11):
     e1.record()
     mfmhd.synchronize()
     ms = e0.elapsed_ms(e1) / steps
-    for L in lv:
+    walks = [L["ctx"].walk() for L in state]
+    for L in state:
         L["ctx"].close()
         L["buf"].free()
-    return {"patches": per_level * levels, "levels": levels,
-            "edges": "8/16/32 (seeded)", "cells": int(total_cells),
+    return {"patches": sum(len(L["boxes"]) for L in lv), "levels": levels,
+            "layout": "nested, ratio 2, level L+1 refines the central half of level L",
+            "edges": "8/16/32 uniform (seed %d)" % seed, "blobs": len(blobs),
+            "cells": int(total_cells), "graphs": True, "walks": walks,
             "ms_per_step": ms, "cell_updates_per_s": total_cells / (ms * 1e-3)}
 
 

@@ -55,20 +55,16 @@ def smooth_state(p, shape, lo, seed=0, amp=0.3, b0=1.0):
 
 
 def periodic_fill(U, ng, dims=3):
-    """One-box periodic ghost fill on a numpy FAB array (nc, nz, ny, nx)."""
+    """One-box periodic ghost fill on a numpy FAB array (nc, nz, ny, nx):
+    every ghost cell takes the valid cell congruent to it modulo the valid
+    extent (which may be shorter than ng)."""
     V = U
-    if dims >= 1:
-        n = V.shape[3] - 2 * ng
-        V[:, :, :, :ng] = V[:, :, :, n:n + ng]
-        V[:, :, :, n + ng:] = V[:, :, :, ng:2 * ng]
-    if dims >= 2:
-        n = V.shape[2] - 2 * ng
-        V[:, :, :ng, :] = V[:, :, n:n + ng, :]
-        V[:, :, n + ng:, :] = V[:, :, ng:2 * ng, :]
-    if dims >= 3:
-        n = V.shape[1] - 2 * ng
-        V[:, :ng, :, :] = V[:, n:n + ng, :, :]
-        V[:, n + ng:, :, :] = V[:, ng:2 * ng, :, :]
+    for d in range(dims):
+        ax = 3 - d
+        m = V.shape[ax]
+        n = m - 2 * ng
+        src = ng + (np.arange(m) - ng) % n
+        V[...] = np.take(V, src, axis=ax)
     return V
 
 
