@@ -321,6 +321,16 @@ int mfm_level_update_dev(mfm_ctx* ctx, const mfm_patch* table, int n,
                          double* new_dt, int* min_cell,
                          double* level_dt, int* min_patch, void* stream);
 
+/* The host checks mfm_level_update_dev makes of a patch table (parameters,
+ * ncomp, ghosts, unused dimensions, the 2^31-cell limit per component, the
+ * domain, the optional F / divB FABs), without a device and without
+ * reading any data pointer (no counterpart in the reference, whose patch
+ * integrator asserts inside the update).  MFM_OK, or the code the update
+ * would return; *bad_patch = the first refused patch (-1 if none) and why
+ * (nullable, why_len bytes) the reason.                                    */
+int mfm_level_check(const mfm_params* p, const mfm_patch* table, int n, double dt,
+                    double dx, int* bad_patch, char* why, size_t why_len);
+
 /* ---- initial data and ghost fill helpers (caller-side utilities) ------- */
 /* HELIOINIT (swlism/HeliosphericF.ChF:1150-1361) / ORSZAGINIT
  * (problems/OrszagF.ChF:27-79) / RIEMANNINIT on a device FAB over `box`. */

@@ -260,6 +260,46 @@ struct PatchGeo {
   size_t off_up, off_w, off_w2, off_f[3], off_bn[3];
 };
 
+// The host checks of one patch of a batch (dimension, ghosts, sizes, the
+// domain and the optional outputs), before any device work; *why is a
+// string literal, so a refusal allocates nothing.
+int check_patch(const Consts& C0, const Consts& Cq, int ng, const PatchIn& P, const char** why) {
+  const int dims = C0.dims;
+  if (!(P.dt > 0.0) || !(P.dx > 0.0)) { *why = "dt and dx must be positive"; return MFM_ERR_ARG; }
+  if (!P.U.data) { *why = "U.data is NULL"; return MFM_ERR_ARG; }
+  if (P.U.ncomp != C0.nc) { *why = "U.ncomp does not match the equation system"; return MFM_ERR_SHAPE; }
+  HBox ub = hb(P.U.box), v = hb(P.valid);
+  if (v.size() <= 0) { *why = "empty valid box"; return MFM_ERR_SHAPE; }
+  for (int d = dims; d < 3; ++d)
+    if (ub.lo[d] != ub.hi[d] || v.lo[d] != v.hi[d] || ub.lo[d] != v.lo[d]) {
+      *why = "unused dimensions must be a single cell";
+      return MFM_ERR_SHAPE;
+    }
+  if (!contains(ub, grow(v, ng, dims))) {
+    *why = "U box must contain the valid box grown by numGhostCells";
+    return MFM_ERR_SHAPE;
+  }
+  if (ub.size() >= (1L << 31)) { *why = "a FAB component must hold fewer than 2^31 cells"; return MFM_ERR_SHAPE; }
+  for (int d = 0; d < dims; ++d)
+    if (Cq.phys[d] && (v.lo[d] < Cq.dom_lo[d] || v.hi[d] > Cq.dom_hi[d])) {
+      *why = "valid box extends outside the problem domain";
+      return MFM_ERR_SHAPE;
+    }
+  if (P.F) {
+    for (int d = 0; d < dims; ++d) {
+      HBox fo = hb(P.F[d].box), want = v;
+      want.hi[d] += 1;
+      if (!P.F[d].data || P.F[d].ncomp != C0.nc || !contains(fo, want)) {
+        *why = "F[d] must cover the faces of the valid box with ncomp components";
+        return MFM_ERR_SHAPE;
+      }
+      if (fo.size() >= (1L << 31)) { *why = "a FAB component must hold fewer than 2^31 cells"; return MFM_ERR_SHAPE; }
+    }
+  }
+  if (P.divB && (!P.divB->data || P.divB->ncomp < 1)) { *why = "divB must have one component"; return MFM_ERR_SHAPE; }
+  return MFM_OK;
+}
+
 // One batched update of n patches.  Each patch names a row of the
 // constants table Ct (nct rows) and carries its own dt, dx and time; all
 // rows share the launch class of c (dimension, fluids, GLM, time scheme,
@@ -289,21 +329,10 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     const PatchIn& P = in[q];
     if (P.cidx < 0 || P.cidx >= nct) return fail(c, MFM_ERR_ARG, "bad constants row");
     const Consts& Cq = Ct[P.cidx];
-    if (!(P.dt > 0.0) || !(P.dx > 0.0)) return fail(c, MFM_ERR_ARG, "dt and dx must be positive");
-    if (!P.U.data) return fail(c, MFM_ERR_ARG, "U.data is NULL");
-    if (P.U.ncomp != C0.nc) return fail(c, MFM_ERR_SHAPE, "U.ncomp does not match the equation system");
+    const char* why = nullptr;
+    const int rc = check_patch(C0, Cq, ng, P, &why);
+    if (rc != MFM_OK) return fail(c, rc, why);
     HBox ub = hb(P.U.box), v = hb(P.valid);
-    if (v.size() <= 0) return fail(c, MFM_ERR_SHAPE, "empty valid box");
-    for (int d = dims; d < 3; ++d)
-      if (ub.lo[d] != ub.hi[d] || v.lo[d] != v.hi[d] || ub.lo[d] != v.lo[d])
-        return fail(c, MFM_ERR_SHAPE, "unused dimensions must be a single cell");
-    if (!contains(ub, grow(v, ng, dims)))
-      return fail(c, MFM_ERR_SHAPE, "U box must contain the valid box grown by numGhostCells");
-    if (ub.size() >= (1L << 31))
-      return fail(c, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^31 cells");
-    for (int d = 0; d < dims; ++d)
-      if (Cq.phys[d] && (v.lo[d] < Cq.dom_lo[d] || v.hi[d] > Cq.dom_hi[d]))
-        return fail(c, MFM_ERR_SHAPE, "valid box extends outside the problem domain");
     PatchGeo& g = G[q];
     g.ub = ub;
     g.vb2 = clip(Cq, v);
@@ -322,18 +351,6 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
       g.off_f[d] = off; off += align256((size_t)g.fb[d].size() * C0.nc * fs);
       g.off_bn[d] = off; off += align256((size_t)g.fb[d].size() * fs);
     }
-    if (P.F) {
-      for (int d = 0; d < dims; ++d) {
-        HBox fo = hb(P.F[d].box), want = v;
-        want.hi[d] += 1;
-        if (!P.F[d].data || P.F[d].ncomp != C0.nc || !contains(fo, want))
-          return fail(c, MFM_ERR_SHAPE, "F[d] must cover the faces of the valid box with ncomp components");
-        if (fo.size() >= (1L << 31))
-          return fail(c, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^31 cells");
-      }
-    }
-    if (P.divB && (!P.divB->data || P.divB->ncomp < 1))
-      return fail(c, MFM_ERR_SHAPE, "divB must have one component");
   }
   // region lists; faces on a physical boundary go to their own list
   const PatchIn* pin = in;
@@ -1166,6 +1183,31 @@ int mfm_level_update_dev(mfm_ctx* c, const mfm_patch* table, int n, double dt,
     in[q] = PatchIn{table[q].U, table[q].valid, table[q].F, table[q].divB, 0, dt, dx, time};
   return run_batch(c, &c->C, 1, in.data(), n, new_dt, min_cell, level_dt, min_patch,
                    (hipStream_t)stream);
+}
+
+int mfm_level_check(const mfm_params* p, const mfm_patch* table, int n, double dt,
+                    double dx, int* bad_patch, char* why, size_t why_len) {
+  if (bad_patch) *bad_patch = -1;
+  if (why && why_len) why[0] = 0;
+  if (!p || !table || n <= 0) return MFM_ERR_ARG;
+  Consts C;
+  std::string w;
+  int rc = build_consts(p, C, w);
+  if (rc == MFM_OK) {
+    const int ng = mfm_num_ghost_cells(p);
+    for (int q = 0; q < n; ++q) {
+      const PatchIn P{table[q].U, table[q].valid, table[q].F, table[q].divB, 0, dt, dx, 0.0};
+      const char* m = nullptr;
+      rc = check_patch(C, C, ng, P, &m);
+      if (rc != MFM_OK) {
+        if (bad_patch) *bad_patch = q;
+        w = m;
+        break;
+      }
+    }
+  }
+  if (rc != MFM_OK && why && why_len) std::snprintf(why, why_len, "%s", w.c_str());
+  return rc;
 }
 
 int mfm_patch_update_host(mfm_ctx* c, mfm_fab U, mfm_box valid, double dt,

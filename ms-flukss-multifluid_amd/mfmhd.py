@@ -94,7 +94,8 @@ EXPORTS = [
     "mfm_ctx_set_dedner", "mfm_ctx_last_error", "mfm_patch_update_dev",
     "mfm_patch_update_noalloc_dev",
     "mfm_patch_update_host", "mfm_patch_compute_dt_dev",
-    "mfm_level_update_dev", "mfm_init_problem_dev", "mfm_fill_periodic_dev",
+    "mfm_level_update_dev", "mfm_level_check", "mfm_init_problem_dev",
+    "mfm_fill_periodic_dev",
     "mfm_dev_count", "mfm_dev_alloc", "mfm_dev_free", "mfm_dev_memcpy",
     "mfm_dev_memset", "mfm_dev_synchronize", "mfm_event_create",
     "mfm_event_destroy", "mfm_event_record", "mfm_event_elapsed_ms",
@@ -159,6 +160,8 @@ def load(path=LIB_PATH):
         C.c_double, C.c_double, ip]
     lib.mfm_num_components.argtypes = [P]
     lib.mfm_hbm_ceiling.argtypes = [C.c_int, C.c_size_t, C.c_int, dp, dp]
+    lib.mfm_level_check.argtypes = [P, C.c_void_p, C.c_int, C.c_double, C.c_double, ip,
+                                    C.c_char_p, C.c_size_t]
     lib.mfm_num_ghost_cells.argtypes = [P]
     lib.mfm_ctx_create.argtypes = [P, C.POINTER(C.c_void_p)]
     lib.mfm_ctx_destroy.argtypes = [C.c_void_p]
@@ -652,6 +655,20 @@ def fill_time_coef(time, t_coarse_new, dt_coarse, dt_fine, ref_ratio):
 # ---- device memory and timing on the library's runtime ---------------------
 def device_count():
     return load().mfm_dev_count()
+
+
+def level_check(params, table, dt, dx):
+    """mfm_level_check: the update's host checks of a patch table
+    [(Fab, Box)], without a device -> (code, first refused patch, reason)."""
+    arr = (PatchRec * len(table))()
+    for q, (f, b) in enumerate(table):
+        arr[q].U = f
+        arr[q].valid = b
+    bad = C.c_int()
+    why = C.create_string_buffer(256)
+    rc = load().mfm_level_check(C.byref(params), C.cast(arr, C.c_void_p), len(table),
+                                float(dt), float(dx), C.byref(bad), why, 256)
+    return rc, bad.value, why.value.decode()
 
 
 CEIL_READ, CEIL_COPY, CEIL_TRIAD = range(3)

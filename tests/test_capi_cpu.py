@@ -133,9 +133,49 @@ def test_ctx_validation_without_device():
     p.precision = mfmhd.PREC_FP64
     p.limiter[4] = mfmhd.LIM_NONE
     assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
+    p = mfmhd.default_params()            # GLM cleaning is single-fluid here
+    p.dedner = 1
+    p.eight_wave = 0
+    p.nfluids = 2
+    assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
+    for flux in (5, 11, 13, 101, 103):
+        # VFRoe, the 7-wave Roe solver without GLM, Dedner through the flux
+        # number (src/RiemannSolver.cpp:40-112): refused, not silently mapped
+        p = mfmhd.default_params()
+        p.flux = flux
+        assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_ARG, flux
     if mfmhd.device_count() == 0:
         p = mfmhd.default_params()
         assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_DEVICE
+
+
+def test_level_check_refuses_without_a_device():
+    """mfm_level_check runs the update's host checks of a patch table with
+    no device and without reading data pointers: an oversized FAB (2^31
+    cells per component or more), missing ghosts, a wrong ncomp, a valid box
+    outside the domain and bad dt / dx are refused with the code the update
+    returns, naming the first bad patch; a good table passes."""
+    p = mfmhd.orszag_params(5.0 / 3.0)
+    ng = mfmhd.num_ghost_cells(p)
+    nc = mfmhd.num_components(p)
+    ok = (mfmhd.fab(0x1000, (-ng,) * 3, (15 + ng,) * 3, nc), mfmhd.Box.of((0,) * 3, (15,) * 3))
+    assert mfmhd.level_check(p, [ok], 0.01, 0.05)[0] == mfmhd.OK
+    n = 1300                                   # 1308^3 > 2^31 cells
+    big = (mfmhd.fab(0x1000, (-ng,) * 3, (n - 1 + ng,) * 3, nc), mfmhd.Box.of((0,) * 3, (n - 1,) * 3))
+    rc, bad, why = mfmhd.level_check(p, [ok, big], 0.01, 0.05)
+    assert rc == mfmhd.ERR_SHAPE and bad == 1 and "2^31" in why
+    thin = (mfmhd.fab(0x1000, (-ng + 1,) * 3, (15 + ng,) * 3, nc), mfmhd.Box.of((0,) * 3, (15,) * 3))
+    rc, bad, why = mfmhd.level_check(p, [thin, ok], 0.01, 0.05)
+    assert rc == mfmhd.ERR_SHAPE and bad == 0 and "numGhostCells" in why
+    wrong = (mfmhd.fab(0x1000, (-ng,) * 3, (15 + ng,) * 3, nc + 1), ok[1])
+    assert mfmhd.level_check(p, [wrong], 0.01, 0.05)[:2] == (mfmhd.ERR_SHAPE, 0)
+    assert mfmhd.level_check(p, [ok], 0.0, 0.05)[0] == mfmhd.ERR_ARG
+    assert mfmhd.level_check(p, [ok], 0.01, -1.0)[0] == mfmhd.ERR_ARG
+    q = mfmhd.helio_params(dx=1.0 / 16, num_cells=(16, 16, 16))
+    out = (mfmhd.fab(0x1000, (-ng,) * 3, (19 + ng,) * 3, 8), mfmhd.Box.of((4,) * 3, (19,) * 3))
+    q.nfluids = 1
+    rc, bad, why = mfmhd.level_check(q, [out], 0.01, 1.0 / 16)
+    assert rc == mfmhd.ERR_SHAPE and "domain" in why
 
 
 def test_binding_constants_match_the_header():
