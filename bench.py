@@ -863,11 +863,10 @@ def main():
                          "(512^2, 1000 steps) runs")
     ap.add_argument("--no-fill", action="store_true",
                     help="skip the level ghost-fill measurement")
-    ap.add_argument("--fused-stage", action="store_true",
-                    help="RK2 stages as the fused stage kernel (k_stage) instead of the "
-                         "separate flux kernels + k_update")
+    ap.add_argument("--stage", choices=["separate", "summed", "fused"], default=None,
+                    help="kernel structure of the RK2 stages (default: the library's)")
     ap.add_argument("--tz", type=int, default=16,
-                    help="planes per z-chunk of the fused stage kernel")
+                    help="planes per z-chunk of the tile kernels (summed, fused)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the run, launch the library's copy kernel over MIB MiB "
                          "twice (calibrates HBM byte counters under rocprofv3 --pmc)")
@@ -883,7 +882,10 @@ def main():
     n = args.n
     p, dx = c3_params(n, prec)
     ctx = mfmhd.Context(p)
-    ctx.set_stage_kernel(args.fused_stage, args.tz)
+    modes = {"separate": mfmhd.STAGE_SEPARATE, "summed": mfmhd.STAGE_SUMMED,
+             "fused": mfmhd.STAGE_FUSED}
+    if args.stage is not None:
+        ctx.set_stage_kernel(modes[args.stage], args.tz)
     nc = ctx.ncomp
     ng = ctx.nghost
     lo, hi = (-ng,) * 3, (n - 1 + ng,) * 3
@@ -940,7 +942,7 @@ def main():
     # per stage, is roofline.kernel.
     step_bytes = 5 * nc * 8
     stage_bytes = {0: 2 * nc * 8, 1: 3 * nc * 8}
-    kind = mfmhd.KT_STAGE if args.fused_stage else mfmhd.KT_UPDATE
+    kind = mfmhd.KT_STAGE if args.stage == "fused" else mfmhd.KT_UPDATE
     kname = mfmhd.KT_NAMES[kind]
     tot_bytes = tot_ms = 0.0
     launches = 0

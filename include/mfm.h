@@ -214,7 +214,9 @@ enum {
   MFM_KT_FLUX_BC = 3,      /* faces on a physical boundary (fluxBC)      */
   MFM_KT_UPDATE = 4,       /* flux divergence, sources, floors, CFL      */
   MFM_KT_PREDICT = 5,      /* Hancock predictor (cell-local half step)   */
-  MFM_KT_STAGE = 6         /* fused stage: interior faces + update       */
+  MFM_KT_STAGE = 6,        /* fused stage: interior faces + update       */
+  MFM_KT_FSUM_PLASMA = 7,  /* summed stage: plasma face sums of a tile   */
+  MFM_KT_FSUM_NEUTRAL = 8  /* summed stage: one neutral fluid's sums     */
 };
 int mfm_ctx_set_timing(mfm_ctx* ctx, int on);
 /* hipGraph replay (no counterpart in the reference): when on, the launch
@@ -238,15 +240,22 @@ int mfm_ctx_autotune(mfm_ctx* ctx, mfm_fab U, mfm_box valid, double dt,
 int mfm_ctx_get_timing(mfm_ctx* ctx, int stage, int kind, double* ms,
                        long* launches, double* items);
 /* Kernel structure of RK2 and first-order stages (no counterpart in the
- * reference).  fused = 0 (default): the separate per-direction flux
- * kernels and k_update.  fused = 1: one launch per stage solves the
- * interior face fluxes of a 16 x 16 column tile into LDS and updates its
- * cells, walking z-chunks of tz planes (default 16), after k_prim and the
- * boundary-face kernel; the face fluxes never reach HBM, but the kernel
- * needs more registers than the separate ones and runs at lower occupancy
- * (slower on the 256^3 workload).  Both give the same results; the Hancock
- * scheme and characteristic slopes always use the separate kernels.       */
-int mfm_ctx_set_stage_kernel(mfm_ctx* ctx, int fused, int tz);
+ * reference; every mode gives the reference's results).
+ * MFM_STAGE_SEPARATE (default): per-direction flux kernels writing the face
+ * fluxes, then k_update.  MFM_STAGE_SUMMED: after k_prim and the
+ * boundary-face kernel, one tile kernel per fluid group (the plasma with
+ * psi and the normal B, then each neutral fluid) solves the interior faces
+ * of a 16 x 16 column tile, z-chunks of tz planes (default 16), into LDS and
+ * writes per cell the face-flux differences summed over the directions;
+ * k_update applies those sums (FP32-flux build only; the FP64 build keeps
+ * the reference's order, the directions one at a time, and runs the
+ * separate kernels in this mode).  MFM_STAGE_FUSED: one tile kernel per stage
+ * solves every face and updates the cells; the face fluxes never reach HBM,
+ * but its register budget makes it slower on the 256^3 workload.  The
+ * Hancock scheme and characteristic slopes always use the separate
+ * kernels.                                                                 */
+enum { MFM_STAGE_SEPARATE = 0, MFM_STAGE_FUSED = 1, MFM_STAGE_SUMMED = 2 };
+int mfm_ctx_set_stage_kernel(mfm_ctx* ctx, int mode, int tz);
 
 /* HBM ceilings of the current device, measured with the library's own
  * streaming kernels (no counterpart in the reference, a CPU code): read

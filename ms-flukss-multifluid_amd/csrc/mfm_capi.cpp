@@ -98,9 +98,9 @@ struct RegionBuilder {
     nblk += (int)((cells + kBlk - 1) / kBlk);
     v.push_back(r);
   }
-  // a region of the fused stage kernel: one block per 16 x 16 column tile
-  // and z-chunk of tz planes
-  void add_tiles(int patch, const HBox& b, int tz) {
+  // a region of a tile kernel: one block per tx x ty column tile and
+  // z-chunk of tz planes (k_stage: 16 x 16, k_fsum: 64 x 4)
+  void add_tiles(int patch, const HBox& b, int tz, int tx = 16, int ty = 16) {
     long cells = b.size();
     if (cells <= 0) return;
     items += cells;
@@ -108,7 +108,7 @@ struct RegionBuilder {
     r.patch = patch;
     for (int d = 0; d < 3; ++d) { r.lo[d] = b.lo[d]; r.n[d] = (int)b.n(d); }
     r.blk0 = nblk;
-    nblk += (int)(((b.n(0) + 15) / 16) * ((b.n(1) + 15) / 16) * ((b.n(2) + tz - 1) / tz));
+    nblk += (int)(((b.n(0) + tx - 1) / tx) * ((b.n(1) + ty - 1) / ty) * ((b.n(2) + tz - 1) / tz));
     v.push_back(r);
   }
 };
@@ -257,7 +257,7 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
 // Geometry of one patch's scratch.
 struct PatchGeo {
   HBox ub, vb1, vb2, fb[3];
-  size_t off_up, off_w, off_w2, off_f[3], off_bn[3];
+  size_t off_up, off_w, off_w2, off_d, off_f[3], off_bn[3];
 };
 
 // The host checks of one patch of a batch (dimension, ghosts, sizes, the
@@ -313,9 +313,14 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
   const bool rk2 = p.time_approx == MFM_TIME_RK2;
   const bool han = p.time_approx == MFM_TIME_HANCOCK;
   const int ng = rk2 ? 4 : (han ? 3 : 2);  // PatchMHDAM::numGhostCells
-  // one launch per stage for the interior faces and the update (k_stage);
-  // the Hancock scheme and characteristic slopes keep the separate kernels
-  const bool fused = c->fused && !han && !C0.char_rec;
+  // tile kernels for the interior faces (k_stage, or k_fsum before
+  // k_update); the Hancock scheme and characteristic slopes keep the
+  // separate kernels, and so does the FP64 build for the summed stages: it
+  // applies the directions one at a time, in the reference's order
+  int mode = (han || C0.char_rec) ? MFM_STAGE_SEPARATE : c->stage_mode;
+  if (mode == MFM_STAGE_SUMMED && p.precision != MFM_PREC_FP32_FLUX) mode = MFM_STAGE_SEPARATE;
+  const bool fused = mode == MFM_STAGE_FUSED;
+  const bool summed = mode == MFM_STAGE_SUMMED;
   const int tz = std::max(1, c->stage_tz);
   if (n <= 0) return fail(c, MFM_ERR_ARG, "empty patch table");
   for (int r = 1; r < nct; ++r)
@@ -345,6 +350,9 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     // Hancock: W of the predicted state; fused RK2: W of U*, written by
     // stage 1 while other blocks still read W of U^n
     if (han || (fused && rk2)) off += align256((size_t)ucells * C0.nc * fs);
+    // summed stages: the per-cell face sums (NC components + normal B)
+    g.off_d = off;
+    if (summed) off += align256((size_t)ucells * (C0.nc + 1) * fs);
     for (int d = 0; d < dims; ++d) {
       g.fb[d] = g.vb1;
       g.fb[d].hi[d] += 1;
@@ -355,7 +363,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
   // region lists; faces on a physical boundary go to their own list
   const PatchIn* pin = in;
   RegionBuilder st1_prim, st1_flux[3], st1_fbc[3], st1_upd, st2_prim, st2_flux[3],
-      st2_fbc[3], st2_upd;
+      st2_fbc[3], st2_upd, st1_sum, st2_sum;
   auto add_faces = [&](RegionBuilder& in, RegionBuilder& bc, int q, HBox f, int d) {
     const Consts& Cq = Ct[pin[q].cidx];
     if (Cq.phys[d]) {
@@ -381,6 +389,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
       for (int d = 0; d < dims; ++d) add_faces(st1_flux[d], st1_fbc[d], q, g.fb[d], d);
       if (fused) st1_upd.add_tiles(q, g.vb1, tz);
       else st1_upd.add(q, g.vb1);
+      if (summed) st1_sum.add_tiles(q, g.vb1, tz, kSumTX, kSumTY);
     } else if (han) {
       // predictor: W on its stencil (value box + 2), one kernel on the cells
       st1_prim.add(q, grow(g.vb1, 2, dims));
@@ -432,12 +441,14 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     }
     if (fused) st2_upd.add_tiles(q, g.vb2, tz);
     else st2_upd.add(q, g.vb2);
+    if (summed) st2_sum.add_tiles(q, g.vb2, tz, kSumTX, kSumTY);
   }
-  constexpr int kLists = 16;
+  constexpr int kLists = 18;
   RegionBuilder* lists[kLists] = {&st1_prim, &st1_flux[0], &st1_flux[1], &st1_flux[2],
                                   &st1_fbc[0], &st1_fbc[1], &st1_fbc[2], &st1_upd,
                                   &st2_prim, &st2_flux[0], &st2_flux[1], &st2_flux[2],
-                                  &st2_fbc[0], &st2_fbc[1], &st2_fbc[2], &st2_upd};
+                                  &st2_fbc[0], &st2_fbc[1], &st2_fbc[2], &st2_upd,
+                                  &st1_sum, &st2_sum};
   // table layout inside the arena (after the scratch)
   // fused RK2: a second patch table for stage 2, whose W is the W of U*
   // that stage 1 wrote (the kernels always read W through the table)
@@ -506,6 +517,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
     D.W = A + g.off_w;
     D.W2 = han ? A + g.off_w2 : nullptr;
     D.Wn = (fused && rk2) ? A + g.off_w2 : D.W;
+    D.D = summed ? A + g.off_d : nullptr;
     for (int d = 0; d < 3; ++d) {
       D.ulo[d] = g.ub.lo[d];
       D.un[d] = (int)g.ub.n(d);
@@ -579,7 +591,9 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
       if (d >= dims) { L.flux[d].nblk = 0; L.fbc[d].nblk = 0; }
     }
     L.upd = rl(l0 + 7);
+    L.sum = rl(l0 == 0 ? 16 : 17);
     L.fused = fused ? 1 : 0;
+    L.summed = summed ? 1 : 0;
     return L;
   };
   if (c->timer) c->timer->n = 0;
@@ -591,9 +605,16 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
       const bool f32 = p.precision == MFM_PREC_FP32_FLUX;
       hipError_t e3 = f32 ? launch_stage<float>(C0, ct, tp, L, S, dtp, idp, st, c->timer)
                           : launch_stage<double>(C0, ct, tp, L, S, dtp, idp, st, c->timer);
-      if (e3 != hipSuccess || !L.fused) return e3;
-      return f32 ? launch_stage_fused<float>(C0, ct, tp, L, S, dtp, idp, st, c->timer)
-                 : launch_stage_fused<double>(C0, ct, tp, L, S, dtp, idp, st, c->timer);
+      if (e3 != hipSuccess) return e3;
+      if (L.fused)
+        return f32 ? launch_stage_fused<float>(C0, ct, tp, L, S, dtp, idp, st, c->timer)
+                   : launch_stage_fused<double>(C0, ct, tp, L, S, dtp, idp, st, c->timer);
+      if (!L.summed) return e3;
+      e3 = f32 ? launch_face_sums<float>(C0, ct, tp, L, S, st, c->timer)
+               : launch_face_sums<double>(C0, ct, tp, L, S, st, c->timer);
+      if (e3 != hipSuccess) return e3;
+      return f32 ? launch_update_summed<float>(C0, ct, tp, L, S, dtp, idp, st, c->timer)
+                 : launch_update_summed<double>(C0, ct, tp, L, S, dtp, idp, st, c->timer);
     };
     hipError_t e2;
     if (han) {
@@ -632,7 +653,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
                                   (long long)off_ct, (long long)off_dtp, (long long)off_idp,
                                   (long long)off_dto, (long long)off_ido, dims, p.time_approx,
                                   p.precision, C0.nf, C0.dedner, c->walk_tile,
-                                  c->walk_xcd, fused ? 1 : 0, tz};
+                                  c->walk_xcd, mode, tz};
     for (int l = 0; l < kLists; ++l) {
       key.push_back((long long)off_reg[l]);
       key.push_back((long long)lists[l]->v.size());
@@ -1048,9 +1069,10 @@ int mfm_ctx_set_walk(mfm_ctx* c, int row_tile, int xcd_order) {
   return MFM_OK;
 }
 
-int mfm_ctx_set_stage_kernel(mfm_ctx* c, int fused, int tz) {
+int mfm_ctx_set_stage_kernel(mfm_ctx* c, int mode, int tz) {
   if (!c || tz < 1 || tz > 4096) return MFM_ERR_ARG;
-  c->fused = fused != 0;
+  if (mode < MFM_STAGE_SEPARATE || mode > MFM_STAGE_SUMMED) return MFM_ERR_ARG;
+  c->stage_mode = mode;
   c->stage_tz = tz;
   return MFM_OK;
 }

@@ -35,10 +35,11 @@ struct mfm_ctx {
   // work walk of the region kernels (row tile in (y, z), XCD renumbering)
   int walk_tile = 8;
   int walk_xcd = 1;
-  // RK2 / first-order stages as one fused kernel (k_stage) per stage, with
-  // z-chunks of stage_tz planes per block; off (default): the separate flux
-  // and update kernels
-  bool fused = false;
+  // kernel structure of RK2 / first-order stages (MFM_STAGE_*): separate
+  // per-direction flux kernels, the fused stage kernel (k_stage) or the
+  // face-sum tile kernels (k_fsum) before k_update; the tile kernels walk
+  // z-chunks of stage_tz planes per block
+  int stage_mode = MFM_STAGE_SEPARATE;
   int stage_tz = 16;
   // host staging of the allocation-free entry point (tables and results of
   // one patch); part of the context, so that entry point needs no allocation

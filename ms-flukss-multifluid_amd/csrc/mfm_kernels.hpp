@@ -18,6 +18,9 @@ struct DevPatch {
   void* Wn;         // where the stage writes W of its output (the next
                     // stage's W): W itself, or a second buffer when the
                     // fused stage kernel still reads W from other blocks
+  void* D;          // summed stages: per cell, the face-flux differences
+                    // summed over the directions (NC components, then the
+                    // normal-B difference), index space of U
   void* F[3];       // face fluxes per direction (scratch, flux precision)
   void* Bn[3];      // face-normal B per direction (scratch)
   double* Fo[3];    // optional time-averaged fluxes returned to the caller
@@ -53,9 +56,11 @@ struct RegionList {
 // flux[d]: interior faces of direction d; fbc[d]: faces of direction d on
 // a physical (non-periodic) domain boundary.
 struct StageLists {
-  RegionList prim, flux[3], fbc[3], upd;
+  RegionList prim, flux[3], fbc[3], upd, sum;
   int fused;        // upd is a list of k_stage tiles (interior faces and the
                     // update in one launch); flux[] is then unused
+  int summed;       // sum is a list of tiles whose face sums (D) the face-sum
+                    // kernels form; k_update then reads D; flux[] unused
 };
 
 struct StepArgs {
@@ -77,7 +82,7 @@ struct StepArgs {
 // Optional kernel timing: a pair of events brackets every launch on the
 // launch stream; the host reads them after the step has synchronized.
 enum KernelKind { KK_PRIM = 0, KK_FLUX_PLASMA, KK_FLUX_NEUTRAL, KK_FLUX_BC, KK_UPDATE,
-                  KK_PREDICT, KK_STAGE, KK_COUNT };
+                  KK_PREDICT, KK_STAGE, KK_FSUM_PLASMA, KK_FSUM_NEUTRAL, KK_COUNT };
 struct KernelTimer {
   static constexpr int kMax = 32;
   hipEvent_t ev[2 * kMax];
@@ -108,6 +113,21 @@ template <typename TF>
 hipError_t launch_stage_fused(const Consts& C, const Consts* CT, const DevPatch* P,
                               const StageLists& L, const StepArgs& A, double* dt_part,
                               long* idx_part, hipStream_t s, KernelTimer* T);
+
+// Column tile of the face-sum kernels (k_fsum): long in x, so that the
+// faces a wavefront solves lie along contiguous rows.
+constexpr int kSumTX = 64, kSumTY = 4;
+
+// Summed stages: the face-sum kernels over L.sum (after launch_stage has run
+// k_prim and the boundary-face fluxes), then k_update reading their sums.
+template <typename TF>
+hipError_t launch_face_sums(const Consts& C, const Consts* CT, const DevPatch* P,
+                            const StageLists& L, const StepArgs& A, hipStream_t s,
+                            KernelTimer* T);
+template <typename TF>
+hipError_t launch_update_summed(const Consts& C, const Consts* CT, const DevPatch* P,
+                                const StageLists& L, const StepArgs& A, double* dt_part,
+                                long* idx_part, hipStream_t s, KernelTimer* T);
 
 hipError_t launch_dt_final(const RegionList& L, const double* dt_part,
                            const long* idx_part, double* dt_out, long* idx_out,

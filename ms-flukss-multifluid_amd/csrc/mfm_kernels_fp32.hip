@@ -7,4 +7,7 @@ namespace mfm {
 template hipError_t launch_stage<float>(const Consts&, const Consts*, const DevPatch*,
                                          const StageLists&, const StepArgs&, double*,
                                          long*, hipStream_t, KernelTimer*);
+template hipError_t launch_update_summed<float>(const Consts&, const Consts*, const DevPatch*,
+                                              const StageLists&, const StepArgs&, double*,
+                                              long*, hipStream_t, KernelTimer*);
 }  // namespace mfm

@@ -8,6 +8,9 @@ namespace mfm {
 template hipError_t launch_stage<double>(const Consts&, const Consts*, const DevPatch*,
                                          const StageLists&, const StepArgs&, double*,
                                          long*, hipStream_t, KernelTimer*);
+template hipError_t launch_update_summed<double>(const Consts&, const Consts*, const DevPatch*,
+                                              const StageLists&, const StepArgs&, double*,
+                                              long*, hipStream_t, KernelTimer*);
 
 hipError_t launch_dt_final(const RegionList& L, const double* dt_part,
                            const long* idx_part, double* dt_out, long* idx_out,

@@ -25,6 +25,7 @@ TIME_FIRST_ORDER, TIME_HANCOCK, TIME_RK2 = 0, 1, 2
 COORD_CARTESIAN, COORD_AXISYMMETRIC = 0, 1
 PROBLEM_NONE, PROBLEM_RIEMANN, PROBLEM_HELIO, PROBLEM_SWLISM = 0, 1, 2, 3
 PREC_FP64, PREC_FP32_FLUX = 0, 1
+STAGE_SEPARATE, STAGE_FUSED, STAGE_SUMMED = 0, 1, 2
 ABI_VERSION = 2
 
 
@@ -124,10 +125,10 @@ EXPORTS = [
 REQ_PENDING, REQ_ACTIVE, REQ_DONE, REQ_FAILED = range(4)
 
 # kernel kinds of mfm_ctx_get_timing
-KT_PRIM, KT_FLUX_PLASMA, KT_FLUX_NEUTRAL, KT_FLUX_BC, KT_UPDATE, KT_PREDICT, KT_STAGE = \
-    range(7)
+(KT_PRIM, KT_FLUX_PLASMA, KT_FLUX_NEUTRAL, KT_FLUX_BC, KT_UPDATE, KT_PREDICT, KT_STAGE,
+ KT_FSUM_PLASMA, KT_FSUM_NEUTRAL) = range(9)
 KT_NAMES = ("k_prim", "k_flux_plasma", "k_flux_neutral", "k_flux_bc", "k_update",
-            "k_hancock_pred", "k_stage")
+            "k_hancock_pred", "k_stage", "k_fsum_plasma", "k_fsum_neutral")
 
 _lib = None
 
@@ -408,10 +409,13 @@ class Context:
     def set_walk(self, row_tile, xcd_order):
         self._check(self.lib.mfm_ctx_set_walk(self.h, int(row_tile), int(xcd_order)))
 
-    def set_stage_kernel(self, fused, tz=16):
-        """fused=True: one k_stage launch per stage (interior faces + update);
-        False: the separate flux kernels and k_update (same results)."""
-        self._check(self.lib.mfm_ctx_set_stage_kernel(self.h, 1 if fused else 0, int(tz)))
+    def set_stage_kernel(self, mode, tz=16):
+        """Kernel structure of RK2 / first-order stages: STAGE_SEPARATE,
+        STAGE_FUSED (True) or STAGE_SUMMED; tz = planes per z-chunk of the
+        tile kernels."""
+        if isinstance(mode, bool):
+            mode = STAGE_FUSED if mode else STAGE_SEPARATE
+        self._check(self.lib.mfm_ctx_set_stage_kernel(self.h, int(mode), int(tz)))
 
     def walk(self):
         t, x = C.c_int(), C.c_int()

@@ -58,8 +58,8 @@ def test_fill_periodic_refuses_bad_boxes():
     ctx.close()
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_kernel_timing_counts_launches_and_keeps_results(oracle_mod, fused):
+@pytest.mark.parametrize("mode", [mfmhd.STAGE_SEPARATE, mfmhd.STAGE_FUSED, mfmhd.STAGE_SUMMED])
+def test_kernel_timing_counts_launches_and_keeps_results(oracle_mod, mode):
     n = 16
     p, dx = helio_setup(oracle_mod, n, 4, mfmhd.PREC_FP32_FLUX)
     nc = mfmhd.num_components(p)
@@ -69,20 +69,25 @@ def test_kernel_timing_counts_launches_and_keeps_results(oracle_mod, fused):
     U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
     dt = 0.5 * oracle_mod.compute_dt(p, U, lo, vlo, vhi, dx)[0]
     ctx = mfmhd.Context(p)
-    ctx.set_stage_kernel(fused)
+    ctx.set_stage_kernel(mode)
     plain = gpu_update(ctx, U, lo, vlo, vhi, dt, dx)
     ctx.set_timing(True)
     timed = gpu_update(ctx, U, lo, vlo, vhi, dt, dx)
     assert np.array_equal(plain["U"], timed["U"]) and plain["dt"] == timed["dt"]
     # one RK2 step: one update per stage, the corrector's over the valid box
-    # (the fused stage kernel, or the separate flux kernels and k_update)
+    # (the fused stage kernel, or k_update after the separate flux kernels or
+    # after the face-sum kernels: one plasma and three neutral launches)
+    fused = mode == mfmhd.STAGE_FUSED
     upd, other = (mfmhd.KT_STAGE, mfmhd.KT_UPDATE) if fused else (mfmhd.KT_UPDATE, mfmhd.KT_STAGE)
+    sums = mode == mfmhd.STAGE_SUMMED
     for st in (0, 1):
         ms, launches, items = ctx.timing(st, upd)
         assert launches == 1 and ms > 0.0, (st, ms, launches)
         assert items >= n ** 3
         assert ctx.timing(st, other)[1] == 0
-        assert ctx.timing(st, mfmhd.KT_FLUX_PLASMA)[1] == (0 if fused else 3)
+        assert ctx.timing(st, mfmhd.KT_FLUX_PLASMA)[1] == (3 if mode == mfmhd.STAGE_SEPARATE else 0)
+        assert ctx.timing(st, mfmhd.KT_FSUM_PLASMA)[1] == (1 if sums else 0)
+        assert ctx.timing(st, mfmhd.KT_FSUM_NEUTRAL)[1] == (3 if sums else 0)
         assert ctx.timing(st, mfmhd.KT_FLUX_BC)[1] == 3   # boundary faces of x, y, z
     assert ctx.timing(1, upd)[2] == n ** 3
     assert ctx.timing(0, mfmhd.KT_PRIM)[1] == 1

@@ -32,7 +32,7 @@ def swlism_setup(nf, nx=48, ny=32, time=mfmhd.TIME_RK2, prec=mfmhd.PREC_FP64,
     return p, dx
 
 
-def pair(oracle_mod, p, U, lo, vlo, vhi, dt, dx, fused=False):
+def pair(oracle_mod, p, U, lo, vlo, vhi, dt, dx, fused=mfmhd.STAGE_SEPARATE):
     ctx = mfmhd.Context(p)
     ctx.set_stage_kernel(fused)
     g = gpu_update(ctx, U, lo, vlo, vhi, dt, dx, 0.0, True, bool(p.eight_wave))
@@ -42,7 +42,7 @@ def pair(oracle_mod, p, U, lo, vlo, vhi, dt, dx, fused=False):
     return g, o
 
 
-@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("fused", [mfmhd.STAGE_SEPARATE, mfmhd.STAGE_FUSED, mfmhd.STAGE_SUMMED])
 @pytest.mark.parametrize("time", [mfmhd.TIME_RK2, mfmhd.TIME_FIRST_ORDER])
 @pytest.mark.parametrize("nf", [1, 2, 3, 4])
 def test_swlism_patch_fp64(oracle_mod, nf, time, fused):
