@@ -117,7 +117,13 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
   if (!p || p->abi_version != MFM_ABI_VERSION) { why = "abi_version mismatch"; return MFM_ERR_ARG; }
   if (p->spacedim != 2 && p->spacedim != 3) { why = "spacedim must be 2 or 3"; return MFM_ERR_ARG; }
   if (p->nfluids < 1 || p->nfluids > 4) { why = "nfluids must be 1..4"; return MFM_ERR_UNSUPPORTED; }
-  if (p->dedner && p->nfluids != 1) { why = "Dedner cleaning is single-fluid"; return MFM_ERR_UNSUPPORTED; }
+  // GLM cleaning with neutral fluids (EqSysMHDMF(iDedner, nF), main/MHDAM.cpp:
+  // 776-826): with caller-filled ghosts; the problems' boundary states here
+  // define no psi for neutral runs
+  if (p->dedner && p->nfluids != 1 && p->problem != MFM_PROBLEM_NONE) {
+    why = "GLM cleaning with neutral fluids needs caller-filled ghosts (problem NONE)";
+    return MFM_ERR_UNSUPPORTED;
+  }
   if (p->flux < 1 || p->flux > 4) { why = "flux must be 1..4"; return MFM_ERR_ARG; }
   if (p->dedner && !(p->flux == 1 || p->flux == 3)) {
     why = "divB_cleaning = 5 must be used with flux = 1 or 3";  // main/MHDAM.cpp:1042-1046

@@ -25,7 +25,7 @@ constexpr double kAU = 1.5e13;
 constexpr double kNet2Dens = 1.0e-6, kNet2Pres = 1.0e-8;
 constexpr double kNet3Dens = 1.0e-7, kNet3Pres = 1.0e-9;
 
-constexpr int kMaxComp = 24;  // 8 + 3*5 (+psi for one fluid)
+constexpr int kMaxComp = 24;  // 8 + psi + 3*5
 
 // Compile-time equation system: NF fluids (plasma + NF-1 neutrals), DED = 1
 // adds the GLM psi component.  Every kernel is instantiated per system so

@@ -31,6 +31,9 @@ hipError_t launch_dt_only(const Consts& C, const Consts* CT, const DevPatch* P,
     case 4: MFM_DT_ONLY(2, 0); break;
     case 6: MFM_DT_ONLY(3, 0); break;
     case 8: MFM_DT_ONLY(4, 0); break;
+    case 5: MFM_DT_ONLY(2, 1); break;
+    case 7: MFM_DT_ONLY(3, 1); break;
+    case 9: MFM_DT_ONLY(4, 1); break;
     default: return hipErrorInvalidValue;
   }
 #undef MFM_DT_ONLY

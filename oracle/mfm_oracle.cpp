@@ -2697,7 +2697,7 @@ double compute_dt(const Eq& E, const Fab& U, const Box& box, double dx,
 int check_args(const mfm_params* p) {
   if (!p || p->abi_version != MFM_ABI_VERSION) return MFM_ERR_ARG;
   if (p->nfluids < 1 || p->nfluids > 4) return MFM_ERR_UNSUPPORTED;
-  if (p->dedner && p->nfluids != 1) return MFM_ERR_UNSUPPORTED;
+  if (p->dedner && p->nfluids != 1 && p->problem != MFM_PROBLEM_NONE) return MFM_ERR_UNSUPPORTED;
   if (p->spacedim != 2 && p->spacedim != 3) return MFM_ERR_ARG;
   if (p->time_approx != MFM_TIME_RK2 && p->time_approx != MFM_TIME_FIRST_ORDER &&
       p->time_approx != MFM_TIME_HANCOCK)

@@ -133,8 +133,8 @@ def test_ctx_validation_without_device():
     p.precision = mfmhd.PREC_FP64
     p.limiter[4] = mfmhd.LIM_NONE
     assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
-    p = mfmhd.default_params()            # GLM cleaning is single-fluid here
-    p.dedner = 1
+    p = mfmhd.helio_params(dx=1.0 / 16, num_cells=(16, 16, 16))   # GLM with neutrals:
+    p.dedner = 1                                                  # caller ghosts only
     p.eight_wave = 0
     p.nfluids = 2
     assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED

@@ -508,3 +508,32 @@ def test_work_walk_does_not_change_results():
         assert r[0] == results[0][0]
         assert np.array_equal(r[1], results[0][1])
     ctx.close()
+
+
+@pytest.mark.parametrize("nf", [2, 3, 4])
+@pytest.mark.parametrize("time", [mfmhd.TIME_RK2, mfmhd.TIME_FIRST_ORDER])
+def test_dedner_with_neutral_fluids(oracle_mod, nf, time):
+    """GLM cleaning with neutral fluids (EqSysMHDMF(iDedner, nF),
+    main/MHDAM.cpp:776-826): psi after the plasma, the neutrals from
+    component 9; periodic 3-D and 2-D boxes, Roe and Roe-averaged states,
+    FP64 <= 1e-12 per step, FP32-flux <= 1e-5 relative L1."""
+    print("dedner with", nf, "fluids, time", time, flush=True)
+    for dims, flux in ((3, mfmhd.FLUX_ROE8), (2, mfmhd.FLUX_ROE8_ROEAVG)):
+        p = periodic_params(nf=nf, flux=flux, lim=mfmhd.LIM_MINMOD, dims=dims, dedner=True,
+                            eight=False, time=time)
+        nc = mfmhd.num_components(p)
+        assert nc == 9 + 5 * (nf - 1)
+        ng = mfmhd.num_ghost_cells(p)
+        vlo, vhi = (0, 0, 0), (15, 13, 11 if dims == 3 else 0)
+        lo, hi, shape = grid_shape(nc, vlo, vhi, ng, dims)
+        U = periodic_fill(smooth_state(p, shape, lo, seed=20 + nf), ng, dims)
+        g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 3e-3, 0.05, ch=1.7, cp=1.1,
+                        want_F=True)
+        check64(g, o, lo, vlo, vhi)
+        p.precision = mfmhd.PREC_FP32_FLUX
+        g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 3e-3, 0.05, ch=1.7, cp=1.1)
+        groups = vector_groups(nf, True)
+        groups.append((5, 6, 7, 8))
+        err = l1_rel_vector_scaled(valid_view(g["U"], lo, vlo, vhi),
+                                   valid_view(o["U"], lo, vlo, vhi), groups)
+        assert err.max() <= TOL32_L1, err
