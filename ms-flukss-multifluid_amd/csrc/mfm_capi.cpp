@@ -124,7 +124,13 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
     why = "GLM cleaning with neutral fluids needs caller-filled ghosts (problem NONE)";
     return MFM_ERR_UNSUPPORTED;
   }
-  if (p->flux < 1 || p->flux > 4) { why = "flux must be 1..4"; return MFM_ERR_ARG; }
+  // the factory's cases (src/RiemannSolver.cpp:39-104) but VFRoe (5);
+  // 101/103 are what GLM cleaning makes of 1/3
+  if (!((p->flux >= 0 && p->flux <= 4) || p->flux == 11 || p->flux == 13)) {
+    why = p->flux == 5 ? "flux = 5 (VFRoe) is not supported"
+                       : "flux must be 0..4, 11 or 13 (GLM cleaning: 1 or 3)";
+    return p->flux == 5 ? MFM_ERR_UNSUPPORTED : MFM_ERR_ARG;
+  }
   if (p->dedner && !(p->flux == 1 || p->flux == 3)) {
     why = "divB_cleaning = 5 must be used with flux = 1 or 3";  // main/MHDAM.cpp:1042-1046
     return MFM_ERR_ARG;
@@ -178,8 +184,15 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
   if (!(p->gamma > 1.0)) { why = "gamma must exceed 1"; return MFM_ERR_ARG; }
   if (p->problem == MFM_PROBLEM_RIEMANN) {
     if (p->nfluids != 1) { why = "Riemann problem is single-fluid here"; return MFM_ERR_UNSUPPORTED; }
+    // RIEMANNGS fills x only, and the update converts U to W on the
+    // domain-clipped box (src/PatchMHDAM.cpp:514-538): the W ghosts of a
+    // non-periodic transverse direction stay unset in the reference, so no
+    // result can match it (riemann14.inputs: is_periodic = 0 1 0 in 3-D)
     for (int d = 1; d < p->spacedim; ++d)
-      if (!p->periodic[d]) { why = "Riemann problem: transverse directions must be periodic"; return MFM_ERR_UNSUPPORTED; }
+      if (!p->periodic[d]) {
+        why = "Riemann problem: a non-periodic transverse direction reads unset ghost primitives in the reference";
+        return MFM_ERR_UNSUPPORTED;
+      }
   } else if (p->problem == MFM_PROBLEM_HELIO) {
     if (p->spacedim != 3) { why = "heliospheric problem is 3-D"; return MFM_ERR_UNSUPPORTED; }
   } else if (p->problem == MFM_PROBLEM_SWLISM) {
@@ -202,8 +215,9 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
   C.nc = 8 + C.dedner + 5 * (C.nf - 1);
   C.iPsi = C.dedner ? 8 : -1;
   C.iRhoN = 8 + C.dedner;
-  C.flux_kind = p->dedner ? 2 : ((p->flux == 2 || p->flux == 4) ? 1 : 0);
-  C.iAveraging = (p->flux == 3 || p->flux == 4) ? 1 : 0;
+  const bool seven = p->dedner || p->flux == 11 || p->flux == 13;
+  C.flux_kind = seven ? 2 : ((p->flux == 2 || p->flux == 4) ? 1 : 0);
+  C.iAveraging = (p->flux == 3 || p->flux == 4 || p->flux == 13) ? 1 : 0;
   C.eight_wave = p->eight_wave ? 1 : 0;
   C.divb_method = p->divb_method;
   C.problem = p->problem;

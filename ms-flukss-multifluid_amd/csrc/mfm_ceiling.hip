@@ -111,7 +111,7 @@ extern "C" int mfm_hbm_ceiling(int kind, size_t bytes, int reps, double* gbps,
       hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(kThreads), 0, 0, arr[q], n, 1.0 + q);
     double best = 1e300;
     for (int r = 0; r <= reps && rc == MFM_OK; ++r) {  // launch 0 warms up
-      hipEventRecord(e0, 0);
+      if (hipEventRecord(e0, 0) != hipSuccess) { rc = MFM_ERR_DEVICE; break; }
       if (kind == MFM_CEIL_READ)
         hipLaunchKernelGGL(k_read, dim3(blocks), dim3(kThreads), 0, 0, arr[0], n, part);
       else if (kind == MFM_CEIL_COPY)
@@ -119,10 +119,12 @@ extern "C" int mfm_hbm_ceiling(int kind, size_t bytes, int reps, double* gbps,
       else
         hipLaunchKernelGGL(k_triad, dim3(blocks), dim3(kThreads), 0, 0, arr[0], arr[1],
                            arr[2], 0.5, n);
-      hipEventRecord(e1, 0);
-      if (hipEventSynchronize(e1) != hipSuccess) { rc = MFM_ERR_DEVICE; break; }
       float ms = 0.0f;
-      hipEventElapsedTime(&ms, e0, e1);
+      if (hipEventRecord(e1, 0) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = MFM_ERR_DEVICE;
+        break;
+      }
       if (r > 0 && ms < best) best = ms;
     }
     if (rc == MFM_OK) {
@@ -130,10 +132,10 @@ extern "C" int mfm_hbm_ceiling(int kind, size_t bytes, int reps, double* gbps,
       if (best_ms) *best_ms = best;
     }
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   for (int q = 0; q < 3; ++q)
-    if (arr[q]) hipFree(arr[q]);
-  if (part) hipFree(part);
+    if (arr[q]) (void)hipFree(arr[q]);
+  if (part) (void)hipFree(part);
   return rc;
 }

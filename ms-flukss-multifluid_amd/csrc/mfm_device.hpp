@@ -42,7 +42,7 @@ struct Sys {
 // /eos/, /RiemSol/, /Dedner/, /chargeexcm/ and problem COMMON blocks).
 struct Consts {
   int dims, nf, nc, dedner, iPsi, iRhoN;
-  int flux_kind;      // 0 Roe8, 1 Rusanov, 2 Dedner (Roe7 + GLM)
+  int flux_kind;      // 0 Roe8, 1 Rusanov, 2 Roe7 (+ the GLM block with psi)
   int iAveraging;
   int eight_wave, divb_method, problem;
   int8_t lim[kMaxComp];

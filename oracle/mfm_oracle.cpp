@@ -147,7 +147,7 @@ struct Eq {
     ch = ch_;
     cp = cp_;
     // src/RiemannSolver.cpp:36-105 (SETRSCONST arguments)
-    iAveraging = (p->flux == 3 || p->flux == 4) ? 1 : 0;
+    iAveraging = (p->flux == 3 || p->flux == 4 || p->flux == 13) ? 1 : 0;
     iLaxFriedrix = 0;
   }
   int limiter(int var) const {
@@ -875,8 +875,10 @@ void plasma_flux_face(const Eq& E, const double* wl, const double* wr,
   double PGasR = std::max(E.smallp, wr[r.p]), RhoR = std::max(E.smallr, wr[r.rho]);
   double f[8];
   int flux = E.P->flux;
-  if (E.dedner) {
-    // DednerRS::fluxes: RIEMANN7F then DEDNER2X2F (src/DednerRS.cpp:38-95)
+  if (E.dedner || flux == 11 || flux == 13) {
+    // Roe7WavesRS::fluxes (src/Roe7WavesRS.cpp:38-78), flux 11 / 13 of the
+    // factory (src/RiemannSolver.cpp:76-89); DednerRS::fluxes: RIEMANN7F
+    // then DEDNER2X2F (src/DednerRS.cpp:38-95)
     roe7_core(E, E.iAveraging, E.iLaxFriedrix, PGasL, RhoL, wl[r.n], wl[r.t1],
               wl[r.t2], wl[r.bn], wl[r.bt1], wl[r.bt2], PGasR, RhoR, wr[r.n],
               wr[r.t1], wr[r.t2], wr[r.bn], wr[r.bt1], wr[r.bt2], f);
@@ -2697,6 +2699,9 @@ double compute_dt(const Eq& E, const Fab& U, const Box& box, double dx,
 int check_args(const mfm_params* p) {
   if (!p || p->abi_version != MFM_ABI_VERSION) return MFM_ERR_ARG;
   if (p->nfluids < 1 || p->nfluids > 4) return MFM_ERR_UNSUPPORTED;
+  // the factory's MHD solvers (src/RiemannSolver.cpp:39-104) but VFRoe (5)
+  if (!((p->flux >= 0 && p->flux <= 4) || p->flux == 11 || p->flux == 13))
+    return MFM_ERR_UNSUPPORTED;
   if (p->dedner && p->nfluids != 1 && p->problem != MFM_PROBLEM_NONE) return MFM_ERR_UNSUPPORTED;
   if (p->spacedim != 2 && p->spacedim != 3) return MFM_ERR_ARG;
   if (p->time_approx != MFM_TIME_RK2 && p->time_approx != MFM_TIME_FIRST_ORDER &&
@@ -3078,6 +3083,13 @@ void oracle_flux_dedner(const mfm_params* p, double ch, const double* wl,
   q.nfluids = 1;
   q.flux = averaging ? MFM_FLUX_ROE8_ROEAVG : MFM_FLUX_ROE8;
   Eq E(&q, ch, 1.0);
+  plasma_flux_face(E, wl, wr, idir, out);
+}
+// The plasma solver the patch update selects from p->flux / p->dedner
+// (RiemannSolver::make, src/RiemannSolver.cpp:39-104), at one face.
+void oracle_flux_face(const mfm_params* p, double ch, const double* wl,
+                      const double* wr, int idir, double* out) {
+  Eq E(p, ch, 1.0);
   plasma_flux_face(E, wl, wr, idir, out);
 }
 void oracle_flux_vanleer(const mfm_params* p, const double* wl,

@@ -73,6 +73,9 @@ void oracle_flux_rusanov(const mfm_params* p, const double* wl,
                          const double* wr, int idir, int averaging, double* out);
 void oracle_flux_dedner(const mfm_params* p, double ch, const double* wl,
                         const double* wr, int idir, int averaging, double* out);
+/* the solver the patch update selects from p->flux and p->dedner */
+void oracle_flux_face(const mfm_params* p, double ch, const double* wl,
+                      const double* wr, int idir, double* out);
 void oracle_flux_vanleer(const mfm_params* p, const double* wl,
                          const double* wr, int idir, double* out);
 /* 1-D face values of cell 0 from (w_-1, w_0, w_+1): out[0]=W-, out[1]=W+ */

@@ -53,6 +53,7 @@ def load():
     lib.oracle_flux_rusanov.argtypes = [P, dp, dp, C.c_int, C.c_int, dp]
     lib.oracle_flux_dedner.argtypes = [P, C.c_double, dp, dp, C.c_int, C.c_int, dp]
     lib.oracle_flux_vanleer.argtypes = [P, dp, dp, C.c_int, dp]
+    lib.oracle_flux_face.argtypes = [P, C.c_double, dp, dp, C.c_int, dp]
     lib.oracle_limiter.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp]
     lib.oracle_face_slopes.argtypes = [C.c_int, C.c_double, C.c_double, dp]
     lib.oracle_face_values.argtypes = [P, mfmhd.Fab, C.c_int, mfmhd.Box, mfmhd.Fab,
@@ -185,6 +186,14 @@ def flux_dedner(p, ch, wl, wr, idir, averaging=0):
     load().oracle_flux_dedner(C.byref(p), ch, _ptr(_vec(wl)), _ptr(_vec(wr)),
                               idir, averaging, _ptr(out))
     return out[:9]
+
+
+def flux_face(p, wl, wr, idir, ch=1.0):
+    """The plasma flux the patch update uses for p.flux / p.dedner."""
+    out = np.zeros(32)
+    load().oracle_flux_face(C.byref(p), ch, _ptr(_vec(wl)), _ptr(_vec(wr)), idir,
+                            _ptr(out))
+    return out[:9] if p.dedner else out[:8]
 
 
 def flux_vanleer(p, wl, wr, idir):

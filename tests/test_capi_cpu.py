@@ -138,15 +138,26 @@ def test_ctx_validation_without_device():
     p.eight_wave = 0
     p.nfluids = 2
     assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
-    for flux in (5, 11, 13, 101, 103):
-        # VFRoe, the 7-wave Roe solver without GLM, Dedner through the flux
-        # number (src/RiemannSolver.cpp:40-112): refused, not silently mapped
+    for flux, rc in ((5, mfmhd.ERR_UNSUPPORTED), (101, mfmhd.ERR_ARG), (103, mfmhd.ERR_ARG),
+                     (-1, mfmhd.ERR_ARG), (6, mfmhd.ERR_ARG), (12, mfmhd.ERR_ARG)):
+        # VFRoe, Dedner through the flux number (src/RiemannSolver.cpp:
+        # 40-112) and numbers the factory has no case for: refused, not
+        # silently mapped
         p = mfmhd.default_params()
+        p.flux = flux
+        assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == rc, flux
+    for flux in (0, 2, 4, 11, 13):
+        # GLM cleaning takes flux 1 or 3 (main/MHDAM.cpp:1045)
+        p = mfmhd.orszag_params(5.0 / 3.0)
+        p.dedner = 1
+        p.eight_wave = 0
         p.flux = flux
         assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_ARG, flux
     if mfmhd.device_count() == 0:
-        p = mfmhd.default_params()
-        assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_DEVICE
+        for flux in (0, 1, 2, 3, 4, 11, 13):
+            p = mfmhd.default_params()
+            p.flux = flux
+            assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_DEVICE, flux
 
 
 def test_level_check_refuses_without_a_device():

@@ -114,6 +114,8 @@ def test_fused_helio_matches_separate_kernels(oracle_mod, nf, prec, mode):
     ("rusanov-mc-2f", 2, mfmhd.FLUX_RUSANOV, mfmhd.LIM_MC, False),
     ("roe8avg-vanleer-4f", 4, mfmhd.FLUX_ROE8_ROEAVG, mfmhd.LIM_VANLEER, False),
     ("glm-superbee", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_SUPERBEE, True),
+    ("roe7-mc-2f", 2, mfmhd.FLUX_ROE7, mfmhd.LIM_MC, False),
+    ("roe7avg-minmod-1f", 1, mfmhd.FLUX_ROE7_ROEAVG, mfmhd.LIM_MINMOD, False),
 ], ids=lambda c: c[0])
 @pytest.mark.parametrize("time", [mfmhd.TIME_RK2, mfmhd.TIME_FIRST_ORDER])
 @pytest.mark.parametrize("mode", MODES)
@@ -171,6 +173,8 @@ def test_fused_2d_and_brio_wu(oracle_mod, mode):
     ("glm-superbee", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_SUPERBEE, True, 3),
     ("glm-2d", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_MINMOD, True, 2),
     ("roe8-3f-2d", 3, mfmhd.FLUX_ROE8, mfmhd.LIM_MINMOD, False, 2),
+    ("roe7-2f", 2, mfmhd.FLUX_ROE7, mfmhd.LIM_MINMOD, False, 3),
+    ("roe7avg-1f-2d", 1, mfmhd.FLUX_ROE7_ROEAVG, mfmhd.LIM_MC, False, 2),
 ], ids=lambda c: c[0])
 @pytest.mark.parametrize("time", [mfmhd.TIME_RK2, mfmhd.TIME_FIRST_ORDER])
 def test_summed_fp32_periodic(oracle_mod, case, time):

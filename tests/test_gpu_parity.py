@@ -73,6 +73,9 @@ CASES = [
     ("roe8-weno3yc-1f", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_WENO3YC),
     ("roe8-cd-1f", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_CD),
     ("rusanov-minmod-4f", 4, mfmhd.FLUX_RUSANOV, mfmhd.LIM_MINMOD),
+    ("roe7-mc-2f", 2, mfmhd.FLUX_ROE7, mfmhd.LIM_MC),
+    ("roe7roeavg-minmod-1f", 1, mfmhd.FLUX_ROE7_ROEAVG, mfmhd.LIM_MINMOD),
+    ("default-minmod-3f", 3, mfmhd.FLUX_DEFAULT, mfmhd.LIM_MINMOD),
 ]
 
 
@@ -373,10 +376,13 @@ def test_fp32_flux_path_error(oracle_mod, nf):
     (mfmhd.FLUX_ROE8, True, mfmhd.TIME_RK2, 3),
     (mfmhd.FLUX_ROE8, True, mfmhd.TIME_RK2, 2),
     (mfmhd.FLUX_RUSANOV, False, mfmhd.TIME_HANCOCK, 3),
-    (mfmhd.FLUX_ROE8_ROEAVG, False, mfmhd.TIME_HANCOCK, 3)])
+    (mfmhd.FLUX_ROE8_ROEAVG, False, mfmhd.TIME_HANCOCK, 3),
+    (mfmhd.FLUX_ROE7, False, mfmhd.TIME_RK2, 3),
+    (mfmhd.FLUX_ROE7_ROEAVG, False, mfmhd.TIME_HANCOCK, 3)])
 def test_fp32_flux_solvers(oracle_mod, flux, dedner, time, dims):
     """The FP32-flux kernels of the other Riemann solver families (Rusanov,
-    Roe-averaged states, 7-wave Roe + GLM in 3-D and 2-D), RK2 and Hancock:
+    Roe-averaged states, 7-wave Roe alone and with GLM in 3-D and 2-D), RK2
+    and Hancock:
     per-variable relative L1 error against the FP64 oracle <= 1e-5, as for
     the 8-wave Roe solver."""
     print("fp32 solver", flux, dedner, time, dims, flush=True)

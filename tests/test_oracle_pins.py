@@ -293,6 +293,70 @@ def test_rusanov_closed_form(oracle_mod, d):
         np.testing.assert_allclose(f, e, rtol=1e-12, atol=1e-12)
 
 
+def test_roe7_selection(oracle_mod):
+    """flux 11 / 13 select the 7-wave Roe solver (src/RiemannSolver.cpp:
+    76-89): zero normal-field flux at every face, and where the normal
+    field is continuous the 8-wave solver's divergence wave has zero
+    strength, so the 7-wave flux equals the 8-wave flux on the same means
+    (11 ~ arithmetic, 13 ~ Roe); flux 0 is the factory's default case, the
+    8-wave flux on arithmetic means."""
+    rng = np.random.default_rng(12)
+    p = params()
+    for t in range(60):
+        d = t % 3
+        wl, wr = rand_prim(rng), rand_prim(rng)
+        p.flux = 0
+        f0 = oracle_mod.flux_face(p, wl, wr, d)
+        p.flux = 1
+        assert np.array_equal(f0, oracle_mod.flux_face(p, wl, wr, d))
+        for flux, avg in ((11, 0), (13, 1)):
+            p.flux = flux
+            f7 = oracle_mod.flux_face(p, wl, wr, d)
+            assert f7[5 + d] == 0.0
+            f8 = oracle_mod.flux_roe8(p, wl, wr, d, averaging=avg)
+            assert abs(f8[5 + d]) > 1e-6    # the divergence wave of a jump in B_n
+            wc = wr.copy()
+            wc[5 + d] = wl[5 + d]
+            f7 = oracle_mod.flux_face(p, wl, wc, d)
+            f8 = oracle_mod.flux_roe8(p, wl, wc, d, averaging=avg)
+            np.testing.assert_allclose(f7, f8, rtol=0, atol=1e-13 * np.max(np.abs(f8)))
+
+
+@pytest.mark.parametrize("flux7,flux8", [(11, 1), (13, 3)])
+def test_roe7_patch_with_uniform_normal_field(oracle_mod, flux7, flux8):
+    """A Brio-Wu tube along x (B_x uniform, one periodic cell in y): every
+    face has a continuous normal field, so two RK2 steps with the 7-wave
+    and the 8-wave solver give the same state."""
+    s4 = np.sqrt(4 * np.pi)
+    L = [1.0, 0, 0, 0, 1.0, 0.75 * s4, s4, 0]
+    R = [0.125, 0, 0, 0, 0.1, 0.75 * s4, -s4, 0]
+    n, ng, dx = 64, 4, 1.0 / 64
+    out = []
+    for flux in (flux7, flux8):
+        p = mfmhd.riemann_params(2.0, L, R, 0.5)
+        p.time_approx = mfmhd.TIME_RK2
+        p.flux = flux
+        p.domain_lo[:] = [0, 0, 0]
+        p.domain_hi[:] = [n - 1, 0, 0]
+        p.periodic[:] = [0, 1, 1]
+        p.spacedim = 2
+        lo = (-ng, -ng, 0)
+        U = oracle_mod.init_problem(p, (8, 1, 1 + 2 * ng, n + 2 * ng), lo, lo,
+                                    (n - 1 + ng, ng, 0), dx)
+        dt = 0.4 * oracle_mod.compute_dt(p, U, lo, (0, 0, 0), (n - 1, 0, 0), dx)[0]
+        for _ in range(2):
+            U[:, :, :, :] = U[:, :, ng:ng + 1, :]
+            r = oracle_mod.patch_update(p, U, lo, (0, 0, 0), (n - 1, 0, 0), dt, dx, 0.0,
+                                        want_F=True)
+            assert r["rc"] == 0
+            U = r["U"]
+        out.append((U, r["F"][0]))
+    (U7, F7), (U8, F8) = out
+    assert np.max(np.abs(U7[0] - 1.0)) > 1e-3     # the tube has evolved
+    np.testing.assert_allclose(U7, U8, rtol=0, atol=1e-12 * np.max(np.abs(U8)))
+    assert np.all(F7[5] == 0.0)
+
+
 def test_dedner_glm_block(oracle_mod):
     """DEDNER2X2F: upwinding of (B_n, psi) with speed Ch; the other
     components come from the 7-wave Roe flux, consistent at W_L = W_R."""
