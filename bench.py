@@ -55,8 +55,9 @@ def note(msg):
 # stages), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on the
 # default workload and scaled by what the counters read for the library's
 # own copy kernel over 1 GiB (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); summary
-# in profiles/k_update_helio256_traffic.md.  Valid for that workload only.
-TRAFFIC = {(256, "fp32flux", "k_update"): 1.4068e10}
+# in profiles/k_update_helio256_traffic.md.  Valid for that workload and
+# stage structure only.
+TRAFFIC = {(256, "fp32flux", "separate", "k_update"): 1.4068e10}
 
 
 def c3_params(n, prec):
@@ -942,8 +943,13 @@ def main():
     # per stage, is roofline.kernel.
     step_bytes = 5 * nc * 8
     stage_bytes = {0: 2 * nc * 8, 1: 3 * nc * 8}
-    kind = mfmhd.KT_STAGE if args.stage == "fused" else mfmhd.KT_UPDATE
+    # the stage structure the library ran (MFM_STAGE_AUTO chooses per call)
+    used = lambda k: sum(ctx.timing(st, k)[1] for st in (0, 1))
+    structure = ("fused" if used(mfmhd.KT_STAGE) else
+                 ("summed" if used(mfmhd.KT_FSUM_PLASMA) else "separate"))
+    kind = mfmhd.KT_STAGE if structure == "fused" else mfmhd.KT_UPDATE
     kname = mfmhd.KT_NAMES[kind]
+    tkey = (n, args.precision, structure, kname)
     tot_bytes = tot_ms = 0.0
     launches = 0
     for st in (0, 1):
@@ -972,11 +978,12 @@ def main():
                                "RK2, Roe8, minmod, 8-wave, charge exchange, single patch" % n,
                    "cells": int(cells), "components": nc, "precision": args.precision,
                    "dt_control": "cfl %.2g, initial_cfl %.2g, max_dt_growth %.2f" % (args.cfl, CFL0, DT_GROWTH),
-                   "walk": {"row_tile": walk[0], "xcd_order": walk[1]}},
+                   "walk": {"row_tile": walk[0], "xcd_order": walk[1]},
+                   "stage_kernels": structure},
         "roofline": {"bound": "hbm", "scope": "whole RK2 step, %d B per cell-update" % step_bytes,
                      "achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": step_achieved / HBM_PEAK,
-                     "traffic": TRAFFIC.get((n, args.precision, kname)),
+                     "traffic": TRAFFIC.get(tkey),
                      "kernel": {"name": kname,
                                 "alg_bytes_per_cell": {"stage1": stage_bytes[0],
                                                        "stage2": stage_bytes[1]},
@@ -985,8 +992,8 @@ def main():
                                 "achieved": k_achieved / 1e9,
                                 "frac": k_achieved / HBM_PEAK,
                                 "traffic_over_alg": (
-                                    TRAFFIC[(n, args.precision, kname)] / (tot_bytes / launches)
-                                    if (n, args.precision, kname) in TRAFFIC else None)}},
+                                    TRAFFIC[tkey] / (tot_bytes / launches)
+                                    if tkey in TRAFFIC else None)}},
         "step_hbm_frac": step_achieved / HBM_PEAK,
         "kernels": kern,
         "wall_s_timed": wall,

@@ -246,20 +246,27 @@ int mfm_ctx_get_timing(mfm_ctx* ctx, int stage, int kind, double* ms,
                        long* launches, double* items);
 /* Kernel structure of RK2 and first-order stages (no counterpart in the
  * reference; every mode gives the reference's results).
- * MFM_STAGE_SEPARATE (default): per-direction flux kernels writing the face
+ * MFM_STAGE_SEPARATE: per-direction flux kernels writing the face
  * fluxes, then k_update.  MFM_STAGE_SUMMED: after k_prim and the
  * boundary-face kernel, one tile kernel per fluid group (the plasma with
  * psi and the normal B, then each neutral fluid) solves the interior faces
- * of a 16 x 16 column tile, z-chunks of tz planes (default 16), into LDS and
+ * of a 64 x 4 column tile, z-chunks of tz planes (default 16), into LDS and
  * writes per cell the face-flux differences summed over the directions;
  * k_update applies those sums (FP32-flux build only; the FP64 build keeps
  * the reference's order, the directions one at a time, and runs the
  * separate kernels in this mode).  MFM_STAGE_FUSED: one tile kernel per stage
  * solves every face and updates the cells; the face fluxes never reach HBM,
- * but its register budget makes it slower on the 256^3 workload.  The
- * Hancock scheme and characteristic slopes always use the separate
- * kernels.                                                                 */
-enum { MFM_STAGE_SEPARATE = 0, MFM_STAGE_FUSED = 1, MFM_STAGE_SUMMED = 2 };
+ * but its register budget makes it slower on the 256^3 workload.
+ * MFM_STAGE_AUTO (default): per batch, the summed stages in the FP32-flux
+ * build when the batch's cells fill at least 3/4 of their 64 x 4 column
+ * tiles (12.6 against 14.2 ms per step on the 256^3 workload), else the
+ * separate kernels; the allocation-free entry point always takes the
+ * separate kernels.  AUTO decides per call: in the FP32-flux build a patch
+ * updated alone and in a level can then take different structures (results
+ * equal within the FP32 bar, not bit for bit); a fixed mode gives the same
+ * bits whatever the batch.  The Hancock scheme and characteristic slopes
+ * always use the separate kernels.                                         */
+enum { MFM_STAGE_SEPARATE = 0, MFM_STAGE_FUSED = 1, MFM_STAGE_SUMMED = 2, MFM_STAGE_AUTO = 3 };
 int mfm_ctx_set_stage_kernel(mfm_ctx* ctx, int mode, int tz);
 
 /* HBM ceilings of the current device, measured with the library's own
@@ -475,7 +482,8 @@ int mfm_fill_time_coef(double time, double t_coarse_new, double dt_coarse,
  * the device (mfm_init_problem_dev), then advances every active request by
  * one step (its own dt, see mfm_server_submit) in the same batched launches
  * (continuous batching) and retires those that reached their step count.
- * A request's result is the same alone and in any mix.  A finished request
+ * A request's result is the same alone and in any mix (the server runs the
+ * separate stage kernels whatever the batch).  A finished request
  * keeps its state (and its share of the budget) until released.            */
 typedef struct mfm_server mfm_server;
 enum { MFM_REQ_PENDING = 0, MFM_REQ_ACTIVE = 1, MFM_REQ_DONE = 2, MFM_REQ_FAILED = 3 };

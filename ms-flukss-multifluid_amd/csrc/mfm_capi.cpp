@@ -24,6 +24,7 @@ namespace mfm {
 namespace host {
 
 constexpr int kBlk = 256;
+constexpr double kAutoTileFill = 0.75;  // MFM_STAGE_AUTO: cells / tile slots
 const double eos_mp = 1.67262171e-24, eos_k = 1.3806505e-16;
 
 int fail(mfm_ctx* c, int code, const std::string& m) {
@@ -338,6 +339,24 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
   // separate kernels, and so does the FP64 build for the summed stages: it
   // applies the directions one at a time, in the reference's order
   int mode = (han || C0.char_rec) ? MFM_STAGE_SEPARATE : c->stage_mode;
+  if (mode == MFM_STAGE_AUTO) {
+    // the summed stages where their 64 x 4 column tiles are well filled
+    // (FP32-flux build; the allocation-free entry point keeps the separate
+    // kernels, whose scratch its bound is stated for)
+    mode = MFM_STAGE_SEPARATE;
+    if (p.precision == MFM_PREC_FP32_FLUX && !fixed) {
+      double cells = 0.0, slots = 0.0;
+      for (int q = 0; q < n; ++q) {
+        if (in[q].cidx < 0 || in[q].cidx >= nct) continue;
+        const HBox v = clip(Ct[in[q].cidx], hb(in[q].valid));
+        if (v.size() <= 0) continue;
+        cells += (double)v.size();
+        slots += (double)((v.n(0) + kSumTX - 1) / kSumTX * kSumTX) *
+                 (double)((v.n(1) + kSumTY - 1) / kSumTY * kSumTY) * (double)v.n(2);
+      }
+      if (slots > 0.0 && cells >= kAutoTileFill * slots) mode = MFM_STAGE_SUMMED;
+    }
+  }
   if (mode == MFM_STAGE_SUMMED && p.precision != MFM_PREC_FP32_FLUX) mode = MFM_STAGE_SEPARATE;
   const bool fused = mode == MFM_STAGE_FUSED;
   const bool summed = mode == MFM_STAGE_SUMMED;
@@ -1091,7 +1110,7 @@ int mfm_ctx_set_walk(mfm_ctx* c, int row_tile, int xcd_order) {
 
 int mfm_ctx_set_stage_kernel(mfm_ctx* c, int mode, int tz) {
   if (!c || tz < 1 || tz > 4096) return MFM_ERR_ARG;
-  if (mode < MFM_STAGE_SEPARATE || mode > MFM_STAGE_SUMMED) return MFM_ERR_ARG;
+  if (mode < MFM_STAGE_SEPARATE || mode > MFM_STAGE_AUTO) return MFM_ERR_ARG;
   c->stage_mode = mode;
   c->stage_tz = tz;
   return MFM_OK;

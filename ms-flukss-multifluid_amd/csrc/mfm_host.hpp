@@ -36,10 +36,11 @@ struct mfm_ctx {
   int walk_tile = 8;
   int walk_xcd = 1;
   // kernel structure of RK2 / first-order stages (MFM_STAGE_*): separate
-  // per-direction flux kernels, the fused stage kernel (k_stage) or the
-  // face-sum tile kernels (k_fsum) before k_update; the tile kernels walk
-  // z-chunks of stage_tz planes per block
-  int stage_mode = MFM_STAGE_SEPARATE;
+  // per-direction flux kernels, the fused stage kernel (k_stage), the
+  // face-sum tile kernels (k_fsum) before k_update, or the choice per batch
+  // between separate and summed (AUTO); the tile kernels walk z-chunks of
+  // stage_tz planes per block
+  int stage_mode = MFM_STAGE_AUTO;
   int stage_tz = 16;
   // host staging of the allocation-free entry point (tables and results of
   // one patch); part of the context, so that entry point needs no allocation

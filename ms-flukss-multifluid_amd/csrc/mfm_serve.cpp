@@ -165,6 +165,9 @@ int mfm_server_create(const mfm_params* cls, double hbm_budget_bytes, int max_ac
   mfm_ctx* c = nullptr;
   int rc = mfm_ctx_create(cls, &c);
   if (rc != MFM_OK) return rc;
+  // a request's result may not depend on the requests batched with it:
+  // the kernel structure is fixed instead of chosen per batch (AUTO)
+  c->stage_mode = MFM_STAGE_SEPARATE;
   mfm_server* s = new mfm_server();
   s->ctx = c;
   s->budget = (size_t)hbm_budget_bytes;

@@ -26,7 +26,7 @@ TIME_FIRST_ORDER, TIME_HANCOCK, TIME_RK2 = 0, 1, 2
 COORD_CARTESIAN, COORD_AXISYMMETRIC = 0, 1
 PROBLEM_NONE, PROBLEM_RIEMANN, PROBLEM_HELIO, PROBLEM_SWLISM = 0, 1, 2, 3
 PREC_FP64, PREC_FP32_FLUX = 0, 1
-STAGE_SEPARATE, STAGE_FUSED, STAGE_SUMMED = 0, 1, 2
+STAGE_SEPARATE, STAGE_FUSED, STAGE_SUMMED, STAGE_AUTO = 0, 1, 2, 3
 ABI_VERSION = 2
 
 
@@ -412,8 +412,9 @@ class Context:
 
     def set_stage_kernel(self, mode, tz=16):
         """Kernel structure of RK2 / first-order stages: STAGE_SEPARATE,
-        STAGE_FUSED (True) or STAGE_SUMMED; tz = planes per z-chunk of the
-        tile kernels."""
+        STAGE_FUSED (True), STAGE_SUMMED or STAGE_AUTO (the default: summed
+        where the column tiles are well filled, FP32-flux build); tz =
+        planes per z-chunk of the tile kernels."""
         if isinstance(mode, bool):
             mode = STAGE_FUSED if mode else STAGE_SEPARATE
         self._check(self.lib.mfm_ctx_set_stage_kernel(self.h, int(mode), int(tz)))

@@ -233,3 +233,38 @@ def test_summed_fp32_helio_and_brio_wu(oracle_mod, nf):
     U = oracle_mod.init_problem(p, shape, lo, lo, hi, dx)
     a, b = both(p, U, lo, vlo, vhi, 1e-4, dx, mode=mfmhd.STAGE_SUMMED)
     fp32_l1(a, oracle_mod.patch_update(p, U, lo, vlo, vhi, 1e-4, dx, 0.0), lo, vlo, vhi, 1)
+
+
+def test_auto_mode_choice(oracle_mod):
+    """MFM_STAGE_AUTO, the default: the summed stages where the cells fill
+    at least 3/4 of their 64 x 4 column tiles in the FP32-flux build, the
+    separate kernels otherwise (a 20-wide box, the FP64 build, the
+    allocation-free entry point); each choice meets its bar against the
+    oracle."""
+    def kinds(p, vlo, vhi, noalloc=False):
+        nc = mfmhd.num_components(p)
+        lo, hi, shape = grid_shape(nc, vlo, vhi, 4)
+        U = periodic_fill(smooth_state(p, shape, lo, seed=5), 4)
+        ctx = mfmhd.Context(p)
+        ctx.set_timing(True)
+        g = gpu_update(ctx, U, lo, vlo, vhi, 2e-3, 1.0 / 32, noalloc=noalloc)
+        used = {k: ctx.timing(s, k)[1] for s in (0, 1)
+                for k in (mfmhd.KT_FSUM_PLASMA, mfmhd.KT_FLUX_PLASMA)}
+        ctx.close()
+        o = oracle_mod.patch_update(p, U, lo, vlo, vhi, 2e-3, 1.0 / 32, 0.0)
+        return g, o, lo, used
+    wide = ((0, 0, 0), (63, 15, 5))
+    p = periodic_params(nf=2, prec=mfmhd.PREC_FP32_FLUX)
+    g, o, lo, used = kinds(p, *wide)
+    assert used[mfmhd.KT_FSUM_PLASMA] > 0 and used[mfmhd.KT_FLUX_PLASMA] == 0
+    fp32_l1(g, o, lo, *wide, 2)
+    g, o, lo, used = kinds(p, (0, 0, 0), (19, 15, 5))
+    assert used[mfmhd.KT_FSUM_PLASMA] == 0 and used[mfmhd.KT_FLUX_PLASMA] > 0
+    fp32_l1(g, o, lo, (0, 0, 0), (19, 15, 5), 2)
+    g, o, lo, used = kinds(p, *wide, noalloc=True)
+    assert used[mfmhd.KT_FSUM_PLASMA] == 0 and used[mfmhd.KT_FLUX_PLASMA] > 0
+    fp32_l1(g, o, lo, *wide, 2)
+    p = periodic_params(nf=2)
+    g, o, lo, used = kinds(p, *wide)
+    assert used[mfmhd.KT_FSUM_PLASMA] == 0 and used[mfmhd.KT_FLUX_PLASMA] > 0
+    check64(g, o, lo, *wide)
