@@ -57,7 +57,8 @@ def note(msg):
 # own copy kernel over 1 GiB (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0); summary
 # in profiles/k_update_helio256_traffic.md.  Valid for that workload and
 # stage structure only.
-TRAFFIC = {(256, "fp32flux", "separate", "k_update"): 1.4068e10}
+TRAFFIC = {(256, "fp32flux", "separate", "k_update"): 1.4068e10,
+           (256, "fp32flux", "summed", "k_update"): 1.0368e10}
 
 
 def c3_params(n, prec):
