@@ -596,19 +596,20 @@ def brio_wu_bench(n=800, t_end=0.1, cfl=0.8):
             "cell_updates_per_s": n * steps / (ms * 1e-3)}
 
 
-def orszag_glm_bench(n=512, steps=1000, cfl=0.5):
+def orszag_glm_bench(n=512, steps=1000, cfl=0.5, flux=mfmhd.FLUX_ROE8):
     """The 2-D Orszag-Tang vortex at its stated size: n^2 periodic on
     [0, 1]^2, gamma = 5/3, GLM (Dedner-Roe: flux 1 with divB_cleaning 5,
     8-wave off, Ch = dx / dt_raw and Cp = sqrt(Ch dx) each step as the level
     driver sets them, dt_raw = dx / c_max), minmod, RK2, FP64, CFL 0.5 (the
     reference's Orszag deck; 0.8 is unstable for this unsplit 2-D update),
-    `steps` steps with the periodic ghost fill in each."""
+    `steps` steps with the periodic ghost fill in each.  flux = FLUX_HLLC
+    runs the HLLC flux with the same GLM block instead of the 7-wave Roe."""
     p = mfmhd.orszag_params(5.0 / 3.0)
     p.spacedim = 2
     p.nfluids = 1
     p.dedner = 1
     p.eight_wave = 0
-    p.flux = mfmhd.FLUX_ROE8
+    p.flux = flux
     p.time_approx = mfmhd.TIME_RK2
     p.precision = mfmhd.PREC_FP64
     for v in range(8):
@@ -651,8 +652,9 @@ def orszag_glm_bench(n=512, steps=1000, cfl=0.5):
     ctx.close()
     U.free()
     rate = n * n / (ms * 1e-3)
-    return {"workload": "Orszag-Tang 2-D %d^2 periodic, GLM (Dedner-Roe), minmod, RK2, "
-                        "CFL %.1f, FP64" % (n, cfl),
+    solver = "HLLC" if flux == mfmhd.FLUX_HLLC else "Dedner-Roe"
+    return {"workload": "Orszag-Tang 2-D %d^2 periodic, GLM (%s), minmod, RK2, "
+                        "CFL %.1f, FP64" % (n, solver, cfl),
             "steps": steps, "t_end": t, "ms_per_step": ms, "cell_updates_per_s": rate,
             "roofline_frac": rate * 5 * nc * 8 / HBM_PEAK,
             "alg_bytes_per_cell_update": 5 * nc * 8}
@@ -1032,6 +1034,7 @@ def main():
     if not args.no_small:
         out["brio_wu_800"] = brio_wu_bench()
         out["orszag_glm_512"] = orszag_glm_bench()
+        out["orszag_glm_512_hllc"] = orszag_glm_bench(flux=mfmhd.FLUX_HLLC)
         note("Brio-Wu and Orszag-Tang done")
     out["roofline"]["copy_ceiling_GBps"] = copy_ceiling() / 1e9
     # the library's own streaming kernels over 2 GiB arrays

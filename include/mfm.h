@@ -51,10 +51,13 @@ enum {
 /* 0 is the factory's default case, the 8-wave Roe flux on arithmetic     */
 /* means like 1; 11 and 13 are the 7-wave Roe flux (no B_n wave, zero     */
 /* normal-field flux) on arithmetic and Roe means; GLM cleaning takes 1   */
-/* or 3 and runs the 7-wave flux with the (B_n, psi) block.               */
+/* or 3 and runs the 7-wave flux with the (B_n, psi) block.  21 is the    */
+/* HLLC flux of ideal MHD (S. Li, JCP 203, 2005), which the reference     */
+/* does not have: pinned by its closed forms, not by reference parity;    */
+/* with GLM cleaning the (B_n, psi) block replaces its normal-field flux. */
 enum { MFM_FLUX_DEFAULT = 0, MFM_FLUX_ROE8 = 1, MFM_FLUX_RUSANOV = 2,
        MFM_FLUX_ROE8_ROEAVG = 3, MFM_FLUX_RUSANOV_ROEAVG = 4,
-       MFM_FLUX_ROE7 = 11, MFM_FLUX_ROE7_ROEAVG = 13 };
+       MFM_FLUX_ROE7 = 11, MFM_FLUX_ROE7_ROEAVG = 13, MFM_FLUX_HLLC = 21 };
 /* limiter1D::eLimiters (src/limiter1D.H:17-28)                           */
 enum { MFM_LIM_CD = 0, MFM_LIM_MINMOD = 1, MFM_LIM_SUPERBEE = 2,
        MFM_LIM_VANALBADA = 3, MFM_LIM_MC = 4, MFM_LIM_HARMONIC = 5,

@@ -126,14 +126,16 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
     return MFM_ERR_UNSUPPORTED;
   }
   // the factory's cases (src/RiemannSolver.cpp:39-104) but VFRoe (5);
-  // 101/103 are what GLM cleaning makes of 1/3
-  if (!((p->flux >= 0 && p->flux <= 4) || p->flux == 11 || p->flux == 13)) {
+  // 101/103 are what GLM cleaning makes of 1/3; 21 = HLLC, this library's
+  if (!((p->flux >= 0 && p->flux <= 4) || p->flux == 11 || p->flux == 13 ||
+        p->flux == MFM_FLUX_HLLC)) {
     why = p->flux == 5 ? "flux = 5 (VFRoe) is not supported"
-                       : "flux must be 0..4, 11 or 13 (GLM cleaning: 1 or 3)";
+                       : "flux must be 0..4, 11, 13 or 21 (GLM cleaning: 1, 3 or 21)";
     return p->flux == 5 ? MFM_ERR_UNSUPPORTED : MFM_ERR_ARG;
   }
-  if (p->dedner && !(p->flux == 1 || p->flux == 3)) {
-    why = "divB_cleaning = 5 must be used with flux = 1 or 3";  // main/MHDAM.cpp:1042-1046
+  if (p->dedner && !(p->flux == 1 || p->flux == 3 || p->flux == MFM_FLUX_HLLC)) {
+    // main/MHDAM.cpp:1042-1046 (1 or 3), and this library's HLLC
+    why = "divB_cleaning = 5 must be used with flux = 1, 3 or 21";
     return MFM_ERR_ARG;
   }
   if (p->dedner && p->eight_wave) { why = "divB_cleaning = 5 can not be used with 8wave_use = 1"; return MFM_ERR_ARG; }
@@ -217,7 +219,7 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
   C.iPsi = C.dedner ? 8 : -1;
   C.iRhoN = 8 + C.dedner;
   const bool seven = p->dedner || p->flux == 11 || p->flux == 13;
-  C.flux_kind = seven ? 2 : ((p->flux == 2 || p->flux == 4) ? 1 : 0);
+  C.flux_kind = p->flux == MFM_FLUX_HLLC ? 3 : (seven ? 2 : ((p->flux == 2 || p->flux == 4) ? 1 : 0));
   C.iAveraging = (p->flux == 3 || p->flux == 4 || p->flux == 13) ? 1 : 0;
   C.eight_wave = p->eight_wave ? 1 : 0;
   C.divb_method = p->divb_method;

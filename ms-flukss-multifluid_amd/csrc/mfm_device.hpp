@@ -42,7 +42,7 @@ struct Sys {
 // /eos/, /RiemSol/, /Dedner/, /chargeexcm/ and problem COMMON blocks).
 struct Consts {
   int dims, nf, nc, dedner, iPsi, iRhoN;
-  int flux_kind;      // 0 Roe8, 1 Rusanov, 2 Roe7 (+ the GLM block with psi)
+  int flux_kind;      // 0 Roe8, 1 Rusanov, 2 Roe7, 3 HLLC (+ the GLM block with psi)
   int iAveraging;
   int eight_wave, divb_method, problem;
   int8_t lim[kMaxComp];
@@ -837,6 +837,111 @@ __device__ __forceinline__ void rusanov_flux(const Consts& C,
                    em * (L.by - R.by));
   out[7] = half * ((L.u * L.bz - L.bx * L.w) + (R.u * R.bz - R.bx * R.w) +
                    em * (L.bz - R.bz));
+}
+
+// HLLC flux of ideal MHD (S. Li, J. Comput. Phys. 203 (2005) 344-357), in
+// the reference's Gaussian units (magnetic pressure B^2/8pi, Maxwell
+// stress B B/4pi).  No counterpart in the reference's solvers.  Outer
+// speeds S_L = min(u_L - c_fL, u_R - c_fR), S_R = max(u_L + c_fL,
+// u_R + c_fR) (normal fast speeds); the star region takes the HLL state's
+// field B* = B_hll and contact speed S_M = (rho u)_hll / rho_hll, the total
+// pressure p_T* = rho_L (S_L - u_L)(S_M - u_L) + p_TL - (B_xL^2 - B_x*^2)/4pi,
+// and on each side K
+//   rho*_K = rho_K (S_K - u_K) / (S_K - S_M),   (rho u)*_K = rho*_K S_M,
+//   (rho v_t)*_K = [rho_K v_tK (S_K - u_K) - (B_x* B_t* - B_xK B_tK)/4pi] / (S_K - S_M),
+//   E*_K = [E_K (S_K - u_K) - p_TK u_K + p_T* S_M
+//           + (B_xK (v_K . B_K) - B_x* (v* . B*))/4pi] / (S_K - S_M),
+// v* = (S_M, v_hll, w_hll); F = F_K + S_K (U*_K - U_K) on the side of the
+// contact that holds x/t = 0, F_L or F_R beyond the outer waves.  Inputs
+// clamped to (smallp, smallr) like the other solvers'; out order as
+// roe_flux's.
+template <typename T>
+__device__ __forceinline__ void hllc_flux(const Consts& C, const MState<T>& L,
+                                          const MState<T>& R, T* out) {
+  const T half = T(0.5), zero = T(0);
+  const T g = T(C.gamma), hg = T(C.hgamma);
+  const T c4 = T(k1_4PI), c8 = T(k1_8PI);
+  // conserved states and physical fluxes of both sides
+  T UL[8], UR[8], FL[8], FR[8];
+  T ptL, ptR;
+  {
+    const T b2 = L.bx * L.bx + L.by * L.by + L.bz * L.bz;
+    ptL = L.p + c8 * b2;
+    const T E = L.p / hg + half * L.rho * (L.u * L.u + L.v * L.v + L.w * L.w) + c8 * b2;
+    const T vb = L.u * L.bx + L.v * L.by + L.w * L.bz;
+    UL[0] = L.rho; UL[1] = L.rho * L.u; UL[2] = L.rho * L.v; UL[3] = L.rho * L.w;
+    UL[4] = E; UL[5] = L.bx; UL[6] = L.by; UL[7] = L.bz;
+    FL[0] = UL[1];
+    FL[1] = UL[1] * L.u + ptL - c4 * L.bx * L.bx;
+    FL[2] = UL[1] * L.v - c4 * L.bx * L.by;
+    FL[3] = UL[1] * L.w - c4 * L.bx * L.bz;
+    FL[4] = (E + ptL) * L.u - c4 * L.bx * vb;
+    FL[5] = zero;
+    FL[6] = L.u * L.by - L.v * L.bx;
+    FL[7] = L.u * L.bz - L.w * L.bx;
+  }
+  {
+    const T b2 = R.bx * R.bx + R.by * R.by + R.bz * R.bz;
+    ptR = R.p + c8 * b2;
+    const T E = R.p / hg + half * R.rho * (R.u * R.u + R.v * R.v + R.w * R.w) + c8 * b2;
+    const T vb = R.u * R.bx + R.v * R.by + R.w * R.bz;
+    UR[0] = R.rho; UR[1] = R.rho * R.u; UR[2] = R.rho * R.v; UR[3] = R.rho * R.w;
+    UR[4] = E; UR[5] = R.bx; UR[6] = R.by; UR[7] = R.bz;
+    FR[0] = UR[1];
+    FR[1] = UR[1] * R.u + ptR - c4 * R.bx * R.bx;
+    FR[2] = UR[1] * R.v - c4 * R.bx * R.by;
+    FR[3] = UR[1] * R.w - c4 * R.bx * R.bz;
+    FR[4] = (E + ptR) * R.u - c4 * R.bx * vb;
+    FR[5] = zero;
+    FR[6] = R.u * R.by - R.v * R.bx;
+    FR[7] = R.u * R.bz - R.w * R.bx;
+  }
+  // normal fast speeds: c_f^2 = (a^2 + b^2 + sqrt((a^2 + b^2)^2 - 4 a^2 b_n^2)) / 2
+  auto cfast = [&](const MState<T>& S) {
+    const T a2 = g * S.p / S.rho;
+    const T b2 = c4 * (S.bx * S.bx + S.by * S.by + S.bz * S.bz) / S.rho;
+    const T bn2 = c4 * S.bx * S.bx / S.rho;
+    const T s = a2 + b2;
+    return tsqrt(half * (s + tsqrt(tmax(zero, s * s - T(4) * a2 * bn2))));
+  };
+  const T cfL = cfast(L), cfR = cfast(R);
+  const T SL = tmin(L.u - cfL, R.u - cfR), SR = tmax(L.u + cfL, R.u + cfR);
+  if (SL >= zero) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) out[c] = FL[c];
+    return;
+  }
+  if (SR <= zero) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) out[c] = FR[c];
+    return;
+  }
+  // the HLL state (divisions, not reciprocals: the FP64 build gives the
+  // oracle's bits)
+  T H[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) H[c] = (SR * UR[c] - SL * UL[c] - (FR[c] - FL[c])) / (SR - SL);
+  const T SM = H[1] / H[0];
+  const T vh = H[2] / H[0], wh = H[3] / H[0];
+  const T bx = H[5], by = H[6], bz = H[7];
+  const T ptS = L.rho * (SL - L.u) * (SM - L.u) + ptL - c4 * (L.bx * L.bx - bx * bx);
+  const bool left = SM >= zero;
+  const MState<T>& K = left ? L : R;
+  const T* UK = left ? UL : UR;
+  const T* FK = left ? FL : FR;
+  const T SK = left ? SL : SR, ptK = left ? ptL : ptR;
+  const T sk = SK - K.u, den = SK - SM;
+  T S[8];
+  S[0] = K.rho * sk / den;
+  S[1] = S[0] * SM;
+  S[2] = (K.rho * K.v * sk - c4 * (bx * by - K.bx * K.by)) / den;
+  S[3] = (K.rho * K.w * sk - c4 * (bx * bz - K.bx * K.bz)) / den;
+  S[4] = (UK[4] * sk - ptK * K.u + ptS * SM +
+          c4 * (K.bx * (K.u * K.bx + K.v * K.by + K.w * K.bz) - bx * (SM * bx + vh * by + wh * bz))) /
+         den;
+  S[5] = bx; S[6] = by; S[7] = bz;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) out[c] = FK[c] + SK * (S[c] - UK[c]);
 }
 
 // van Leer flux-vector splitting for one neutral fluid (VANLEERF /

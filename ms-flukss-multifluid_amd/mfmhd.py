@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "libmfmhd.so")
 OK, ERR_ARG, ERR_SHAPE, ERR_DEVICE, ERR_NUMERIC, ERR_NOMEM, ERR_UNSUPPORTED = \
     0, -1, -2, -3, -4, -5, -6
 FLUX_ROE8, FLUX_RUSANOV, FLUX_ROE8_ROEAVG, FLUX_RUSANOV_ROEAVG = 1, 2, 3, 4
-FLUX_DEFAULT, FLUX_ROE7, FLUX_ROE7_ROEAVG = 0, 11, 13
+FLUX_DEFAULT, FLUX_ROE7, FLUX_ROE7_ROEAVG, FLUX_HLLC = 0, 11, 13, 21
 (LIM_CD, LIM_MINMOD, LIM_SUPERBEE, LIM_VANALBADA, LIM_MC, LIM_HARMONIC,
  LIM_HYPERBEE, LIM_KOREN, LIM_VENKAT, LIM_VANLEER, LIM_WENO3, LIM_WENO3YC,
  LIM_NONE) = range(13)

@@ -730,6 +730,77 @@ void roe7_core(const Eq& E, int iAveraging, int iLaxFriedrix, double PGasL,
 }
 
 // RUSANOVF (src/RusanovF.ChF:4-197), after the input clamps.
+// HLLC flux of ideal MHD after S. Li, "An HLLC Riemann solver for
+// magneto-hydrodynamics", J. Comput. Phys. 203 (2005) 344-357, eqs. (14)-
+// (31), written in Gaussian units (B^2/8pi, B B/4pi).  The reference has no
+// HLLC solver: this function is pinned by its closed forms (consistency,
+// exact stationary contacts, upwinding beyond the outer waves, agreement
+// with HLL in the density of a jump without a contact), not by reference
+// parity.  Outer speeds after Davis: S_L = min(u_L - cf_L, u_R - cf_R),
+// S_R = max(u_L + cf_L, u_R + cf_R); star field = the HLL field; contact
+// speed S_M = (rho u)_hll / rho_hll.
+void hllc_core(const Eq& E, double PGasL, double RhoL, double UL, double VL,
+               double WL, double BXL, double BYL, double BZL, double PGasR,
+               double RhoR, double UR, double VR, double WR, double BXR,
+               double BYR, double BZR, double* out) {
+  struct Side {
+    double rho, u, v, w, p, bx, by, bz, pt, U[8], F[8];
+  } s[2] = {{RhoL, UL, VL, WL, PGasL, BXL, BYL, BZL},
+            {RhoR, UR, VR, WR, PGasR, BXR, BYR, BZR}};
+  double cf[2];
+  for (Side& q : s) {
+    const double B2 = q.bx * q.bx + q.by * q.by + q.bz * q.bz;
+    q.pt = q.p + d_1_8PI * B2;
+    const double En = q.p / E.hgamma + half * q.rho * (q.u * q.u + q.v * q.v + q.w * q.w) +
+                      d_1_8PI * B2;
+    const double vB = q.u * q.bx + q.v * q.by + q.w * q.bz;
+    const double cons[8] = {q.rho, q.rho * q.u, q.rho * q.v, q.rho * q.w, En, q.bx, q.by, q.bz};
+    const double flx[8] = {q.rho * q.u,
+                           q.rho * q.u * q.u + q.pt - d_1_4PI * q.bx * q.bx,
+                           q.rho * q.u * q.v - d_1_4PI * q.bx * q.by,
+                           q.rho * q.u * q.w - d_1_4PI * q.bx * q.bz,
+                           (En + q.pt) * q.u - d_1_4PI * q.bx * vB,
+                           zero,
+                           q.u * q.by - q.v * q.bx,
+                           q.u * q.bz - q.w * q.bx};
+    for (int c = 0; c < 8; ++c) { q.U[c] = cons[c]; q.F[c] = flx[c]; }
+  }
+  for (int k = 0; k < 2; ++k) {
+    const Side& q = s[k];
+    const double a2 = E.gamma * q.p / q.rho;
+    const double va2 = d_1_4PI * (q.bx * q.bx + q.by * q.by + q.bz * q.bz) / q.rho;
+    const double vax2 = d_1_4PI * q.bx * q.bx / q.rho;
+    const double disc = std::max(zero, (a2 + va2) * (a2 + va2) - 4.0 * a2 * vax2);
+    cf[k] = std::sqrt(half * (a2 + va2 + std::sqrt(disc)));
+  }
+  const double SL = std::min(s[0].u - cf[0], s[1].u - cf[1]);
+  const double SR = std::max(s[0].u + cf[0], s[1].u + cf[1]);
+  if (SL >= zero) { for (int c = 0; c < 8; ++c) out[c] = s[0].F[c]; return; }
+  if (SR <= zero) { for (int c = 0; c < 8; ++c) out[c] = s[1].F[c]; return; }
+  double hll[8];
+  for (int c = 0; c < 8; ++c)
+    hll[c] = (SR * s[1].U[c] - SL * s[0].U[c] - (s[1].F[c] - s[0].F[c])) / (SR - SL);
+  const double SM = hll[1] / hll[0];
+  const double bxs = hll[5], bys = hll[6], bzs = hll[7];
+  const double vs = hll[2] / hll[0], ws = hll[3] / hll[0];
+  const double pts = s[0].rho * (SL - s[0].u) * (SM - s[0].u) + s[0].pt -
+                     d_1_4PI * (s[0].bx * s[0].bx - bxs * bxs);
+  const int k = SM >= zero ? 0 : 1;
+  const Side& q = s[k];
+  const double S = k == 0 ? SL : SR;
+  const double den = S - SM;
+  double st[8];
+  st[0] = q.rho * (S - q.u) / den;
+  st[1] = st[0] * SM;
+  st[2] = (q.rho * q.v * (S - q.u) - d_1_4PI * (bxs * bys - q.bx * q.by)) / den;
+  st[3] = (q.rho * q.w * (S - q.u) - d_1_4PI * (bxs * bzs - q.bx * q.bz)) / den;
+  st[4] = (q.U[4] * (S - q.u) - q.pt * q.u + pts * SM +
+           d_1_4PI * (q.bx * (q.u * q.bx + q.v * q.by + q.w * q.bz) -
+                      bxs * (SM * bxs + vs * bys + ws * bzs))) / den;
+  st[5] = bxs; st[6] = bys; st[7] = bzs;
+  for (int c = 0; c < 8; ++c) out[c] = q.F[c] + S * (st[c] - q.U[c]);
+}
+
 void rusanov_core(const Eq& E, int iAveraging, double PGasL, double RhoL,
                   double UL, double VL, double WL, double BXL, double BYL,
                   double BZL, double PGasR, double RhoR, double UR, double VR,
@@ -875,7 +946,12 @@ void plasma_flux_face(const Eq& E, const double* wl, const double* wr,
   double PGasR = std::max(E.smallp, wr[r.p]), RhoR = std::max(E.smallr, wr[r.rho]);
   double f[8];
   int flux = E.P->flux;
-  if (E.dedner || flux == 11 || flux == 13) {
+  if (flux == MFM_FLUX_HLLC) {
+    // this library's HLLC (no counterpart in the reference); with GLM the
+    // (B_n, psi) block below replaces its normal-field flux
+    hllc_core(E, PGasL, RhoL, wl[r.n], wl[r.t1], wl[r.t2], wl[r.bn], wl[r.bt1], wl[r.bt2],
+              PGasR, RhoR, wr[r.n], wr[r.t1], wr[r.t2], wr[r.bn], wr[r.bt1], wr[r.bt2], f);
+  } else if (E.dedner || flux == 11 || flux == 13) {
     // Roe7WavesRS::fluxes (src/Roe7WavesRS.cpp:38-78), flux 11 / 13 of the
     // factory (src/RiemannSolver.cpp:76-89); DednerRS::fluxes: RIEMANN7F
     // then DEDNER2X2F (src/DednerRS.cpp:38-95)
@@ -2699,8 +2775,10 @@ double compute_dt(const Eq& E, const Fab& U, const Box& box, double dx,
 int check_args(const mfm_params* p) {
   if (!p || p->abi_version != MFM_ABI_VERSION) return MFM_ERR_ARG;
   if (p->nfluids < 1 || p->nfluids > 4) return MFM_ERR_UNSUPPORTED;
-  // the factory's MHD solvers (src/RiemannSolver.cpp:39-104) but VFRoe (5)
-  if (!((p->flux >= 0 && p->flux <= 4) || p->flux == 11 || p->flux == 13))
+  // the factory's MHD solvers (src/RiemannSolver.cpp:39-104) but VFRoe (5),
+  // and the HLLC flux (21)
+  if (!((p->flux >= 0 && p->flux <= 4) || p->flux == 11 || p->flux == 13 ||
+        p->flux == MFM_FLUX_HLLC))
     return MFM_ERR_UNSUPPORTED;
   if (p->dedner && p->nfluids != 1 && p->problem != MFM_PROBLEM_NONE) return MFM_ERR_UNSUPPORTED;
   if (p->spacedim != 2 && p->spacedim != 3) return MFM_ERR_ARG;

@@ -139,7 +139,8 @@ def test_ctx_validation_without_device():
     p.nfluids = 2
     assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_UNSUPPORTED
     for flux, rc in ((5, mfmhd.ERR_UNSUPPORTED), (101, mfmhd.ERR_ARG), (103, mfmhd.ERR_ARG),
-                     (-1, mfmhd.ERR_ARG), (6, mfmhd.ERR_ARG), (12, mfmhd.ERR_ARG)):
+                     (-1, mfmhd.ERR_ARG), (6, mfmhd.ERR_ARG), (12, mfmhd.ERR_ARG),
+                     (20, mfmhd.ERR_ARG), (22, mfmhd.ERR_ARG)):
         # VFRoe, Dedner through the flux number (src/RiemannSolver.cpp:
         # 40-112) and numbers the factory has no case for: refused, not
         # silently mapped
@@ -147,15 +148,22 @@ def test_ctx_validation_without_device():
         p.flux = flux
         assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == rc, flux
     for flux in (0, 2, 4, 11, 13):
-        # GLM cleaning takes flux 1 or 3 (main/MHDAM.cpp:1045)
+        # GLM cleaning takes flux 1 or 3 (main/MHDAM.cpp:1045), or this
+        # library's HLLC (21)
         p = mfmhd.orszag_params(5.0 / 3.0)
         p.dedner = 1
         p.eight_wave = 0
         p.flux = flux
         assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_ARG, flux
     if mfmhd.device_count() == 0:
-        for flux in (0, 1, 2, 3, 4, 11, 13):
+        for flux in (0, 1, 2, 3, 4, 11, 13, 21):
             p = mfmhd.default_params()
+            p.flux = flux
+            assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_DEVICE, flux
+        for flux in (1, 3, 21):
+            p = mfmhd.orszag_params(5.0 / 3.0)
+            p.dedner = 1
+            p.eight_wave = 0
             p.flux = flux
             assert lib.mfm_ctx_create(C.byref(p), C.byref(h)) == mfmhd.ERR_DEVICE, flux
 

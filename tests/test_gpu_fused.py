@@ -116,6 +116,8 @@ def test_fused_helio_matches_separate_kernels(oracle_mod, nf, prec, mode):
     ("glm-superbee", 1, mfmhd.FLUX_ROE8, mfmhd.LIM_SUPERBEE, True),
     ("roe7-mc-2f", 2, mfmhd.FLUX_ROE7, mfmhd.LIM_MC, False),
     ("roe7avg-minmod-1f", 1, mfmhd.FLUX_ROE7_ROEAVG, mfmhd.LIM_MINMOD, False),
+    ("hllc-minmod-2f", 2, mfmhd.FLUX_HLLC, mfmhd.LIM_MINMOD, False),
+    ("hllc-glm", 1, mfmhd.FLUX_HLLC, mfmhd.LIM_MINMOD, True),
 ], ids=lambda c: c[0])
 @pytest.mark.parametrize("time", [mfmhd.TIME_RK2, mfmhd.TIME_FIRST_ORDER])
 @pytest.mark.parametrize("mode", MODES)
@@ -175,6 +177,8 @@ def test_fused_2d_and_brio_wu(oracle_mod, mode):
     ("roe8-3f-2d", 3, mfmhd.FLUX_ROE8, mfmhd.LIM_MINMOD, False, 2),
     ("roe7-2f", 2, mfmhd.FLUX_ROE7, mfmhd.LIM_MINMOD, False, 3),
     ("roe7avg-1f-2d", 1, mfmhd.FLUX_ROE7_ROEAVG, mfmhd.LIM_MC, False, 2),
+    ("hllc-3f", 3, mfmhd.FLUX_HLLC, mfmhd.LIM_MINMOD, False, 3),
+    ("hllc-glm-2d", 1, mfmhd.FLUX_HLLC, mfmhd.LIM_MINMOD, True, 2),
 ], ids=lambda c: c[0])
 @pytest.mark.parametrize("time", [mfmhd.TIME_RK2, mfmhd.TIME_FIRST_ORDER])
 def test_summed_fp32_periodic(oracle_mod, case, time):

@@ -76,6 +76,8 @@ CASES = [
     ("roe7-mc-2f", 2, mfmhd.FLUX_ROE7, mfmhd.LIM_MC),
     ("roe7roeavg-minmod-1f", 1, mfmhd.FLUX_ROE7_ROEAVG, mfmhd.LIM_MINMOD),
     ("default-minmod-3f", 3, mfmhd.FLUX_DEFAULT, mfmhd.LIM_MINMOD),
+    ("hllc-minmod-2f", 2, mfmhd.FLUX_HLLC, mfmhd.LIM_MINMOD),
+    ("hllc-mc-1f", 1, mfmhd.FLUX_HLLC, mfmhd.LIM_MC),
 ]
 
 
@@ -378,7 +380,10 @@ def test_fp32_flux_path_error(oracle_mod, nf):
     (mfmhd.FLUX_RUSANOV, False, mfmhd.TIME_HANCOCK, 3),
     (mfmhd.FLUX_ROE8_ROEAVG, False, mfmhd.TIME_HANCOCK, 3),
     (mfmhd.FLUX_ROE7, False, mfmhd.TIME_RK2, 3),
-    (mfmhd.FLUX_ROE7_ROEAVG, False, mfmhd.TIME_HANCOCK, 3)])
+    (mfmhd.FLUX_ROE7_ROEAVG, False, mfmhd.TIME_HANCOCK, 3),
+    (mfmhd.FLUX_HLLC, False, mfmhd.TIME_RK2, 3),
+    (mfmhd.FLUX_HLLC, True, mfmhd.TIME_RK2, 2),
+    (mfmhd.FLUX_HLLC, False, mfmhd.TIME_HANCOCK, 3)])
 def test_fp32_flux_solvers(oracle_mod, flux, dedner, time, dims):
     """The FP32-flux kernels of the other Riemann solver families (Rusanov,
     Roe-averaged states, 7-wave Roe alone and with GLM in 3-D and 2-D), RK2
@@ -543,3 +548,27 @@ def test_dedner_with_neutral_fluids(oracle_mod, nf, time):
         err = l1_rel_vector_scaled(valid_view(g["U"], lo, vlo, vhi),
                                    valid_view(o["U"], lo, vlo, vhi), groups)
         assert err.max() <= TOL32_L1, err
+
+
+@pytest.mark.parametrize("dims,dedner,time,nf", [
+    (3, False, mfmhd.TIME_HANCOCK, 4),
+    (3, False, mfmhd.TIME_FIRST_ORDER, 1),
+    (2, True, mfmhd.TIME_RK2, 1),
+    (3, True, mfmhd.TIME_RK2, 2),
+])
+def test_hllc_schemes_fp64(oracle_mod, dims, dedner, time, nf):
+    """The HLLC flux (flux 21, no counterpart in the reference) in the FP64
+    build against the oracle's HLLC: Hancock with four fluids, first order,
+    and with GLM cleaning (its (B_n, psi) block) in 2-D and with a neutral
+    fluid in 3-D."""
+    print("hllc", dims, dedner, time, nf, flush=True)
+    p = periodic_params(nf=nf, flux=mfmhd.FLUX_HLLC, lim=mfmhd.LIM_MINMOD, dims=dims,
+                        dedner=dedner, eight=not dedner, time=time)
+    ng = mfmhd.num_ghost_cells(p)
+    vlo, vhi = (0, 0, 0), (15, 13, 11 if dims == 3 else 0)
+    lo, hi, shape = grid_shape(mfmhd.num_components(p), vlo, vhi, ng, dims)
+    U = periodic_fill(smooth_state(p, shape, lo, seed=21 + nf), ng, dims)
+    g, o = run_pair(oracle_mod, p, U, lo, vlo, vhi, 2e-3, 1.0 / 16, ch=1.7, cp=1.1, want_F=True)
+    check64(g, o, lo, vlo, vhi)
+    for d in range(dims):
+        assert np.max(np.abs(g["F"][d] - o["F"][d])) <= 1e-12 * np.max(np.abs(o["F"][d]))

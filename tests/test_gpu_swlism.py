@@ -71,11 +71,11 @@ def test_swlism_patch_fp64(oracle_mod, nf, time, fused):
 
 
 @pytest.mark.parametrize("flux", [mfmhd.FLUX_ROE8, mfmhd.FLUX_RUSANOV, mfmhd.FLUX_ROE7,
-                                  mfmhd.FLUX_ROE7_ROEAVG])
+                                  mfmhd.FLUX_ROE7_ROEAVG, mfmhd.FLUX_HLLC])
 def test_swlism_other_solvers_fp64(oracle_mod, flux):
-    """The deck's own solver (8-wave Roe on arithmetic means), Rusanov and
-    the 7-wave Roe solver (at the boundary faces too, before SWLISMBC), one
-    step from the initial state at the deck's initial_cfl."""
+    """The deck's own solver (8-wave Roe on arithmetic means), Rusanov, the
+    7-wave Roe solver and the HLLC flux (at the boundary faces too, before
+    SWLISMBC), one step from the initial state at the deck's initial_cfl."""
     p, dx = swlism_setup(2, 48, 32, mfmhd.TIME_FIRST_ORDER, flux=flux)
     nc = mfmhd.num_components(p)
     vlo, vhi = (0, 0, 0), (47, 31, 0)
