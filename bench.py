@@ -139,7 +139,7 @@ def cpu_baseline(n_sample, threads, n_single=32):
             "host": host_cpu()}
 
 
-def serve_bench(prec, n_req=512, steps=20, rate_hz=4096.0, seed=7):
+def serve_bench(prec, n_req=512, steps=20, rate_hz=4096.0, seed=7, stage=None):
     """Ensemble serving of the 2-D axisymmetric SW/LISM problem (the
     reference's swlism2F deck: 1500 AU domain, sun at 600 AU on the axis,
     plasma + 1 neutral = 13 components, first order in time, cfl 0.25,
@@ -151,7 +151,7 @@ def serve_bench(prec, n_req=512, steps=20, rate_hz=4096.0, seed=7):
     positivity at the wind / LISM interface of SWLISMINIT's state within a
     few steps).  Reports the p50/p99 per-request step latency (one server
     iteration advances every active request one step) beside the
-    throughput."""
+    throughput.  stage: the server's stage structure (default: its own)."""
     rng = np.random.default_rng(seed)
     reqs = []
     for _ in range(n_req):
@@ -168,6 +168,8 @@ def serve_bench(prec, n_req=512, steps=20, rate_hz=4096.0, seed=7):
         reqs.append((p, dx, nx * ny))
     arrivals = np.cumsum(rng.exponential(1.0 / rate_hz, size=n_req))
     srv = mfmhd.Server(reqs[0][0], 64.0 * (1 << 30))
+    if stage is not None:
+        srv.set_stage_kernel(stage)
     t0 = time.perf_counter()
     nxt, ids, meta = 0, [], {}
     cells = 0.0
@@ -199,6 +201,8 @@ def serve_bench(prec, n_req=512, steps=20, rate_hz=4096.0, seed=7):
                         "U[350, 800] km/s, lismB ~ U[2, 5] uG, 2 fluids (13 comps), "
                         "first order, Roe8 on Roe averages, minmod",
             "arrival": "Poisson %.0f /s" % rate_hz,
+            "stage_kernels": {None: "server default (separate)", mfmhd.STAGE_SEPARATE: "separate",
+                              mfmhd.STAGE_SUMMED: "summed"}.get(stage, str(stage)),
             "p50_step_ms": st["p50_ms"], "p99_step_ms": st["p99_ms"],
             "iterations": st["iterations"],
             "cell_updates": cells,

@@ -485,8 +485,8 @@ int mfm_fill_time_coef(double time, double t_coarse_new, double dt_coarse,
  * the device (mfm_init_problem_dev), then advances every active request by
  * one step (its own dt, see mfm_server_submit) in the same batched launches
  * (continuous batching) and retires those that reached their step count.
- * A request's result is the same alone and in any mix (the server runs the
- * separate stage kernels whatever the batch).  A finished request
+ * A request's result is the same alone and in any mix (the server runs one
+ * fixed stage structure whatever the batch).  A finished request
  * keeps its state (and its share of the budget) until released.            */
 typedef struct mfm_server mfm_server;
 enum { MFM_REQ_PENDING = 0, MFM_REQ_ACTIVE = 1, MFM_REQ_DONE = 2, MFM_REQ_FAILED = 3 };
@@ -494,6 +494,13 @@ int mfm_server_create(const mfm_params* cls, double hbm_budget_bytes,
                       int max_active, mfm_server** out);
 int mfm_server_destroy(mfm_server* srv);
 const char* mfm_server_last_error(const mfm_server* srv);
+/* The stage structure of every batch (MFM_STAGE_SEPARATE, the default,
+ * MFM_STAGE_SUMMED or MFM_STAGE_FUSED; see mfm_ctx_set_stage_kernel).  A
+ * fixed structure keeps a request's result independent of its batch, so
+ * MFM_STAGE_AUTO is refused; so is a change after the first request (its
+ * HBM estimate depends on the structure).  No counterpart in the
+ * reference.                                                              */
+int mfm_server_set_stage_kernel(mfm_server* srv, int mode);
 /* Step control as the AMR driver's (AMRLevelIdealMHD::computeNewDt,
  * computeInitialDt): first dt = initial_cfl * dt_raw, then
  * min(cfl * dt_raw, max_dt_growth * previous dt); dt_raw = dx / c_max. */

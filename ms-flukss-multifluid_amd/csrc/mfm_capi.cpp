@@ -218,8 +218,7 @@ int build_consts(const mfm_params* p, Consts& C, std::string& why) {
   C.nc = 8 + C.dedner + 5 * (C.nf - 1);
   C.iPsi = C.dedner ? 8 : -1;
   C.iRhoN = 8 + C.dedner;
-  const bool seven = p->dedner || p->flux == 11 || p->flux == 13;
-  C.flux_kind = p->flux == MFM_FLUX_HLLC ? 3 : (seven ? 2 : ((p->flux == 2 || p->flux == 4) ? 1 : 0));
+  C.flux_kind = flux_kind_of(*p);
   C.iAveraging = (p->flux == 3 || p->flux == 4 || p->flux == 13) ? 1 : 0;
   C.eight_wave = p->eight_wave ? 1 : 0;
   C.divb_method = p->divb_method;

@@ -81,6 +81,16 @@ int fail(mfm_ctx* c, int code, const std::string& m);
 int hipfail(mfm_ctx* c, hipError_t e, const char* what);
 int build_consts(const mfm_params* p, Consts& C, std::string& why);
 
+// The plasma solver family of p (Consts::flux_kind): the kernels are
+// compiled per family, and one launch serves one family.  0: 8-wave Roe
+// (flux 0, 1, 3), 1: Rusanov (2, 4), 2: 7-wave Roe (11, 13, and GLM with
+// 1 or 3), 3: HLLC (21, with or without GLM).
+inline int flux_kind_of(const mfm_params& p) {
+  if (p.flux == MFM_FLUX_HLLC) return 3;
+  if (p.dedner || p.flux == 11 || p.flux == 13) return 2;
+  return (p.flux == 2 || p.flux == 4) ? 1 : 0;
+}
+
 // Caller-provided memory for run_batch instead of the context's growing
 // arena and staging (the allocation-free entry point): dev holds the patch
 // scratch, tables and reductions; host the staged tables and results.

@@ -67,14 +67,20 @@ int sfail(mfm_server* s, int code, const std::string& m) {
 
 // Device bytes of one request: its FAB plus the scratch run_batch allocates
 // for it (U*, W, face fluxes and normal B of the first-stage value box).
-size_t request_bytes(const mfm_params& cls, const Consts& C, const HBox& fab, const HBox& valid) {
+size_t request_bytes(const mfm_params& cls, int mode, const Consts& C, const HBox& fab,
+                     const HBox& valid) {
   const int dims = C.dims;
   const size_t fs = cls.precision == MFM_PREC_FP32_FLUX ? 4 : 8;
   const size_t cells = (size_t)fab.size();
+  const bool tiles = cls.time_approx != MFM_TIME_HANCOCK && !cls.char_reconstruction;
   size_t b = align256(cells * C.nc * 8);          // U
   b += align256(cells * C.nc * 8);                // U*
   b += align256(cells * C.nc * fs);               // W
   if (cls.time_approx == MFM_TIME_HANCOCK) b += align256(cells * C.nc * fs);  // W of U_pred
+  if (tiles && mode == MFM_STAGE_FUSED && cls.time_approx == MFM_TIME_RK2)
+    b += align256(cells * C.nc * fs);             // W of U*
+  if (tiles && mode == MFM_STAGE_SUMMED && cls.precision == MFM_PREC_FP32_FLUX)
+    b += align256(cells * (C.nc + 1) * fs);       // face sums
   HBox v1 = cls.time_approx == MFM_TIME_RK2 ? grow(valid, 2, dims)
             : (cls.time_approx == MFM_TIME_HANCOCK ? grow(valid, 1, dims) : valid);
   for (int d = 0; d < dims; ++d) {
@@ -86,7 +92,7 @@ size_t request_bytes(const mfm_params& cls, const Consts& C, const HBox& fab, co
 }
 
 // Riemann solver family (the plasma flux kernel is compiled per family)
-int solver(const mfm_params& p) { return p.dedner ? 2 : ((p.flux == 2 || p.flux == 4) ? 1 : 0); }
+int solver(const mfm_params& p) { return flux_kind_of(p); }
 
 bool same_class(const mfm_params& a, const mfm_params& b) {
   return a.spacedim == b.spacedim && a.coord_sys == b.coord_sys &&
@@ -188,6 +194,16 @@ int mfm_server_destroy(mfm_server* s) {
   return MFM_OK;
 }
 
+int mfm_server_set_stage_kernel(mfm_server* s, int mode) {
+  if (!s) return MFM_ERR_ARG;
+  if (mode != MFM_STAGE_SEPARATE && mode != MFM_STAGE_SUMMED && mode != MFM_STAGE_FUSED)
+    return sfail(s, MFM_ERR_ARG, "the server takes a fixed stage structure (not AUTO)");
+  if (!s->all.empty())
+    return sfail(s, MFM_ERR_ARG, "set the stage structure before the first request");
+  s->ctx->stage_mode = mode;
+  return MFM_OK;
+}
+
 const char* mfm_server_last_error(const mfm_server* s) {
   if (!s) return "null server";
   return s->err.c_str();
@@ -228,7 +244,7 @@ int mfm_server_submit(mfm_server* s, const mfm_params* p, double dx, int nsteps,
   r->cfl0 = initial_cfl;
   r->growth = max_dt_growth;
   r->nsteps = nsteps;
-  r->bytes = request_bytes(s->ctx->p, r->C, r->fab, r->valid);
+  r->bytes = request_bytes(s->ctx->p, s->ctx->stage_mode, r->C, r->fab, r->valid);
   r->id = s->next_id++;
   s->all[r->id] = r;
   s->pending.push_back(r);

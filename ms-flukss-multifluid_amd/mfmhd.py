@@ -105,6 +105,7 @@ EXPORTS = [
     "mfm_ctx_set_walk", "mfm_ctx_get_walk", "mfm_ctx_autotune",
     "mfm_ctx_set_stage_kernel", "mfm_hbm_ceiling",
     "mfm_server_create", "mfm_server_destroy", "mfm_server_last_error",
+    "mfm_server_set_stage_kernel",
     "mfm_server_submit", "mfm_server_step", "mfm_server_request_info",
     "mfm_server_fetch", "mfm_server_release", "mfm_server_stats",
     "mfm_server_request_error", "mfm_fill_plan_create",
@@ -200,6 +201,7 @@ def load(path=LIB_PATH):
     lib.mfm_fill_periodic_dev.argtypes = [C.c_void_p, Fab, Box, C.c_void_p]
     vp, dbl, lng, it = C.c_void_p, C.c_double, C.c_long, C.c_int
     lib.mfm_server_create.argtypes = [P, dbl, it, C.POINTER(vp)]
+    lib.mfm_server_set_stage_kernel.argtypes = [vp, it]
     lib.mfm_server_destroy.argtypes = [vp]
     lib.mfm_server_last_error.argtypes = [vp]
     lib.mfm_server_last_error.restype = C.c_char_p
@@ -808,6 +810,11 @@ class Server:
         if rc != OK:
             msg = self.lib.mfm_server_last_error(self.h)
             raise MfmError(rc, msg.decode() if msg else "")
+
+    def set_stage_kernel(self, mode):
+        """Stage structure of every batch: STAGE_SEPARATE (default),
+        STAGE_SUMMED or STAGE_FUSED; before the first request."""
+        self._check(self.lib.mfm_server_set_stage_kernel(self.h, int(mode)))
 
     def submit(self, params, dx, nsteps, cfl, initial_cfl=None, max_dt_growth=1.0e30):
         """Queue a solve; dt control as the AMR driver's (first step

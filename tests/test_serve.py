@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import mfmhd
-from helpers import rel_err_per_comp, valid_view
+from helpers import l1_rel_vector_scaled, rel_err_per_comp, valid_view, vector_groups
 
 pytestmark = pytest.mark.gpu
 
@@ -213,7 +213,9 @@ def test_swlism_request_alone_equals_request_in_mix():
 
 def test_server_refuses_another_class():
     """A 2-D axisymmetric request is not of a 3-D heliospheric server's
-    class (nor the other way round)."""
+    class (nor the other way round), and a request of another solver family
+    (7-wave Roe, HLLC, Rusanov) is not of an 8-wave Roe server's; the
+    averaging of one family may differ (a per-request constant)."""
     ph, dxh = req_params(16)
     ps, dxs = swlism_req(40)
     srv = mfmhd.Server(ph, 8 << 30)
@@ -223,4 +225,49 @@ def test_server_refuses_another_class():
     srv = mfmhd.Server(ps, 8 << 30)
     with pytest.raises(mfmhd.MfmError):
         srv.submit(ph, dxh, 1, 0.5)
+    for flux in (mfmhd.FLUX_ROE7, mfmhd.FLUX_ROE7_ROEAVG, mfmhd.FLUX_HLLC, mfmhd.FLUX_RUSANOV):
+        q, dq = swlism_req(40)
+        q.flux = flux
+        with pytest.raises(mfmhd.MfmError):
+            srv.submit(q, dq, 1, 0.25)
+    q, dq = swlism_req(40)
+    q.flux = mfmhd.FLUX_ROE8
+    srv.submit(q, dq, 1, 0.25)
     srv.close()
+
+
+def test_server_stage_structure():
+    """The stage structure is fixed per server: AUTO is refused, and so is a
+    change after the first request; with the summed stages (FP32-flux
+    build) a request is still the same alone and in a mix, bit for bit,
+    and within the FP32 bar of the separate kernels' result."""
+    srv = mfmhd.Server(swlism_req(40)[0], 8 << 30)
+    with pytest.raises(mfmhd.MfmError):
+        srv.set_stage_kernel(mfmhd.STAGE_AUTO)
+    p, dx = swlism_req(40)
+    srv.submit(p, dx, 1, 0.25)
+    with pytest.raises(mfmhd.MfmError):
+        srv.set_stage_kernel(mfmhd.STAGE_SUMMED)
+    srv.close()
+    pa, dxa = swlism_req(64, sunV=6.0e7, lismB=2.5e-6, prec=mfmhd.PREC_FP32_FLUX)
+    res = {}
+    for mode in (mfmhd.STAGE_SUMMED, mfmhd.STAGE_SEPARATE):
+        s1 = mfmhd.Server(pa, 8 << 30)
+        s1.set_stage_kernel(mode)
+        a = s1.submit(pa, dxa, 4, 0.25, 0.1, 1.02)
+        run_until_done(s1, lambda: [a])
+        res[mode] = s1.fetch(a)
+        s1.close()
+    s2 = mfmhd.Server(pa, 8 << 30)
+    s2.set_stage_kernel(mfmhd.STAGE_SUMMED)
+    ids = [s2.submit(*swlism_req(m, sunV=v, lismB=b, prec=mfmhd.PREC_FP32_FLUX), 6, 0.25, 0.1, 1.02)
+           for m, v, b in ((72, 3.5e7, 5e-6), (96, 8.0e7, 2e-6))]
+    a = s2.submit(pa, dxa, 4, 0.25, 0.1, 1.02)
+    run_until_done(s2, lambda: ids + [a])
+    Um = s2.fetch(a)
+    s2.close()
+    assert np.array_equal(res[mfmhd.STAGE_SUMMED], Um)
+    # two FP32 results, each within 1e-5 of the FP64 one (vector norm)
+    err = l1_rel_vector_scaled(res[mfmhd.STAGE_SUMMED], res[mfmhd.STAGE_SEPARATE],
+                               vector_groups(2))
+    assert err.max() <= 2e-5, err
