@@ -346,7 +346,7 @@ int mfm_level_update_dev(mfm_ctx* ctx, const mfm_patch* table, int n,
                          double* level_dt, int* min_patch, void* stream);
 
 /* The host checks mfm_level_update_dev makes of a patch table (parameters,
- * ncomp, ghosts, unused dimensions, the 2^31-cell limit per component, the
+ * ncomp, ghosts, unused dimensions, the 2^30-cell limit per component, the
  * domain, the optional F / divB FABs), without a device and without
  * reading any data pointer (no counterpart in the reference, whose patch
  * integrator asserts inside the update).  MFM_OK, or the code the update

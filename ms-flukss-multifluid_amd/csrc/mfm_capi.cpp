@@ -301,7 +301,7 @@ int check_patch(const Consts& C0, const Consts& Cq, int ng, const PatchIn& P, co
     *why = "U box must contain the valid box grown by numGhostCells";
     return MFM_ERR_SHAPE;
   }
-  if (ub.size() >= (1L << 31)) { *why = "a FAB component must hold fewer than 2^31 cells"; return MFM_ERR_SHAPE; }
+  if (ub.size() >= kMaxFabCells) { *why = "a FAB component must hold fewer than 2^30 cells"; return MFM_ERR_SHAPE; }
   for (int d = 0; d < dims; ++d)
     if (Cq.phys[d] && (v.lo[d] < Cq.dom_lo[d] || v.hi[d] > Cq.dom_hi[d])) {
       *why = "valid box extends outside the problem domain";
@@ -315,7 +315,7 @@ int check_patch(const Consts& C0, const Consts& Cq, int ng, const PatchIn& P, co
         *why = "F[d] must cover the faces of the valid box with ncomp components";
         return MFM_ERR_SHAPE;
       }
-      if (fo.size() >= (1L << 31)) { *why = "a FAB component must hold fewer than 2^31 cells"; return MFM_ERR_SHAPE; }
+      if (fo.size() >= kMaxFabCells) { *why = "a FAB component must hold fewer than 2^30 cells"; return MFM_ERR_SHAPE; }
     }
   }
   if (P.divB && (!P.divB->data || P.divB->ncomp < 1)) { *why = "divB must have one component"; return MFM_ERR_SHAPE; }
@@ -779,7 +779,7 @@ int run_dt(mfm_ctx* c, const Consts& C0, const mfm_fab& U, const mfm_box& box,
   // the region list below without its one region)
   if (b.size() == 0) return fail(c, MFM_ERR_SHAPE, "empty box");
   if (!contains(ub, b)) return fail(c, MFM_ERR_SHAPE, "box outside U");
-  if (ub.size() >= (1L << 31)) return fail(c, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^31 cells");
+  if (ub.size() >= kMaxFabCells) return fail(c, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^30 cells");
   if (!(dx > 0.0)) return fail(c, MFM_ERR_ARG, "dx must be positive");
   RegionBuilder rb;
   rb.add(0, b);

@@ -76,6 +76,9 @@ inline bool contains(const HBox& a, const HBox& b) {
   return true;
 }
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+// Cells per FAB component the update kernels take: their 32-bit cell
+// offsets, and the 32-bit byte offsets of their 4-byte arrays, cannot wrap.
+constexpr long kMaxFabCells = 1L << 30;
 
 int fail(mfm_ctx* c, int code, const std::string& m);
 int hipfail(mfm_ctx* c, hipError_t e, const char* what);

@@ -235,9 +235,9 @@ int mfm_server_submit(mfm_server* s, const mfm_params* p, double dx, int nsteps,
     return sfail(s, MFM_ERR_SHAPE, "empty problem domain");
   }
   r->fab = grow(r->valid, ng, dims);
-  if (r->fab.size() >= (1L << 31)) {
+  if (r->fab.size() >= kMaxFabCells) {
     delete r;
-    return sfail(s, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^31 cells");
+    return sfail(s, MFM_ERR_SHAPE, "a FAB component must hold fewer than 2^30 cells");
   }
   r->dx = dx;
   r->cfl = cfl;

@@ -170,8 +170,8 @@ def test_ctx_validation_without_device():
 
 def test_level_check_refuses_without_a_device():
     """mfm_level_check runs the update's host checks of a patch table with
-    no device and without reading data pointers: an oversized FAB (2^31
-    cells per component or more), missing ghosts, a wrong ncomp, a valid box
+    no device and without reading data pointers: an oversized FAB (2^30
+    cells per component or more; one cell fewer passes), missing ghosts, a wrong ncomp, a valid box
     outside the domain and bad dt / dx are refused with the code the update
     returns, naming the first bad patch; a good table passes."""
     p = mfmhd.orszag_params(5.0 / 3.0)
@@ -179,10 +179,14 @@ def test_level_check_refuses_without_a_device():
     nc = mfmhd.num_components(p)
     ok = (mfmhd.fab(0x1000, (-ng,) * 3, (15 + ng,) * 3, nc), mfmhd.Box.of((0,) * 3, (15,) * 3))
     assert mfmhd.level_check(p, [ok], 0.01, 0.05)[0] == mfmhd.OK
-    n = 1300                                   # 1308^3 > 2^31 cells
+    n = 1024 - 2 * ng                          # a FAB of 1024^3 = 2^30 cells
     big = (mfmhd.fab(0x1000, (-ng,) * 3, (n - 1 + ng,) * 3, nc), mfmhd.Box.of((0,) * 3, (n - 1,) * 3))
     rc, bad, why = mfmhd.level_check(p, [ok, big], 0.01, 0.05)
-    assert rc == mfmhd.ERR_SHAPE and bad == 1 and "2^31" in why
+    assert rc == mfmhd.ERR_SHAPE and bad == 1 and "2^30" in why
+    # 1023 x 1024 x 1024 cells, one plane of x fewer: accepted
+    fits = (mfmhd.fab(0x1000, (-ng,) * 3, (n - 2 + ng, n - 1 + ng, n - 1 + ng), nc),
+            mfmhd.Box.of((0,) * 3, (n - 2, n - 1, n - 1)))
+    assert mfmhd.level_check(p, [ok, fits], 0.01, 0.05)[0] == mfmhd.OK
     thin = (mfmhd.fab(0x1000, (-ng + 1,) * 3, (15 + ng,) * 3, nc), mfmhd.Box.of((0,) * 3, (15,) * 3))
     rc, bad, why = mfmhd.level_check(p, [thin, ok], 0.01, 0.05)
     assert rc == mfmhd.ERR_SHAPE and bad == 0 and "numGhostCells" in why
