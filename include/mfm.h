@@ -262,7 +262,7 @@ int mfm_ctx_get_timing(mfm_ctx* ctx, int stage, int kind, double* ms,
  * but its register budget makes it slower on the 256^3 workload.
  * MFM_STAGE_AUTO (default): per batch, the summed stages in the FP32-flux
  * build when the batch's cells fill at least 3/4 of their 64 x 4 column
- * tiles (12.6 against 14.2 ms per step on the 256^3 workload), else the
+ * tiles (12.4 against 13.9-14.2 ms per step on the 256^3 workload), else the
  * separate kernels; the allocation-free entry point always takes the
  * separate kernels.  AUTO decides per call: in the FP32-flux build a patch
  * updated alone and in a level can then take different structures (results
