@@ -428,11 +428,13 @@ def test_shape_errors_leave_state_untouched():
     ctx.close()
 
 
-def test_graph_replay_matches_eager():
+@pytest.mark.parametrize("mode", [None, mfmhd.STAGE_SUMMED])
+def test_graph_replay_matches_eager(mode):
     """hipGraph replay of the update (mfm_ctx_set_graphs) gives the eager
     results bit for bit on every replay, with eager work and
     synchronizations between replays, and recaptures when the table changes
-    shape (one patch, then a two-patch level, then one patch again)."""
+    shape (one patch, then a two-patch level, then one patch again); with
+    the default stage structure and with the summed stages."""
     n = 20
     dx = 1.0 / n
     p = mfmhd.helio_params(lismB=3e-6, lismBdir=(2 ** -0.5, 2 ** -0.5, 0.0), lismV=2.6e6,
@@ -445,6 +447,9 @@ def test_graph_replay_matches_eager():
     vlo, vhi = (0, 0, 0), (n - 1,) * 3
     lo, hi, shape = grid_shape(nc, vlo, vhi, 4)
     eager, graph = mfmhd.Context(p), mfmhd.Context(p)
+    if mode is not None:
+        eager.set_stage_kernel(mode)
+        graph.set_stage_kernel(mode)
     graph.set_graphs(True)
     Ue = mfmhd.DeviceArray(shape)
     fe = mfmhd.fab(Ue.ptr, lo, hi, nc)
