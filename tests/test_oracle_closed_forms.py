@@ -231,6 +231,73 @@ def test_limiters_textbook(oracle_mod, name, lim):
     assert n == 400
 
 
+def test_hyperbee_is_half_the_power_mean(oracle_mod):
+    """The reference's hyperbee slope (src/reconstructionF.ChF:430-464) is
+    half the power mean of order -1/2 of the one-sided differences,
+    1/2 ((|D-|^(-1/2) + |D+|^(-1/2)) / 2)^(-2) with their common sign
+    (the harmonic limiter is the order -1 mean, minmod the order -inf one);
+    checked against scipy's power mean on monotone triples with slope
+    ratios from 1/100 to 100."""
+    from scipy.stats import pmean
+    rng = np.random.default_rng(41)
+    for _ in range(300):
+        w0 = rng.uniform(-2, 2)
+        sgn = rng.choice([-1.0, 1.0])
+        dm, dp = sgn * 10 ** rng.uniform(-2, 1, 2)
+        s = 0.5 * sgn * pmean([abs(dm), abs(dp)], -0.5)
+        left, right = oracle_mod.limiter(mfmhd.LIM_HYPERBEE, w0 - dm, w0, w0 + dp)
+        scale = abs(w0) + abs(dm) + abs(dp)
+        assert abs(left - (w0 - s)) <= 1e-13 * scale
+        assert abs(right - (w0 + s)) <= 1e-13 * scale
+        # a mean: between the one-sided half differences
+        assert 0.5 * min(abs(dm), abs(dp)) - 1e-15 <= abs(s) <= 0.5 * max(abs(dm), abs(dp)) + 1e-15
+
+
+def test_venkatakrishnan_is_kappa_third_times_van_albada(oracle_mod):
+    """The reference's Venkatakrishnan form (src/reconstructionF.ChF:950-988)
+    is the unlimited kappa = 1/3 increment 1/4 [(1-k) D_up + (1+k) D_down]
+    towards each face, scaled by van Albada's phi(r) = (r^2 + r)/(r^2 + 1)
+    of the ratio r = D_down / D_up (D_down the difference across the face,
+    D_up the one behind it); zero at extrema.  Slopes >= 1e-2 keep the
+    reference's 1e-10 regularisation below 1e-7 of the slope."""
+    k = 1.0 / 3.0
+    va = lambda r: (r * r + r) / (r * r + 1)
+    rng = np.random.default_rng(42)
+    for _ in range(300):
+        w0 = rng.uniform(-2, 2)
+        sgn = rng.choice([-1.0, 1.0])
+        dm, dp = sgn * 10 ** rng.uniform(-2, 1, 2)
+        right = w0 + va(dp / dm) * 0.25 * ((1 - k) * dm + (1 + k) * dp)
+        left = w0 - va(dm / dp) * 0.25 * ((1 - k) * dp + (1 + k) * dm)
+        got = oracle_mod.limiter(mfmhd.LIM_VENKAT, w0 - dm, w0, w0 + dp)
+        scale = abs(dm) + abs(dp)
+        assert abs(got[0] - left) <= 1e-7 * scale
+        assert abs(got[1] - right) <= 1e-7 * scale
+        # non-monotone data: first order
+        assert oracle_mod.limiter(mfmhd.LIM_VENKAT, w0 + dm, w0, w0 + dp) == (w0, w0)
+
+
+def test_centred_face_values_are_linear(oracle_mod):
+    """CDFACEVALUES (src/reconstructionF.ChF:1056-1079) is linear in the
+    three cell values, so linearity on random triples plus its values on
+    the unit triples pin it completely: the lo-face value W- is the mean of
+    the cell and its +1 neighbour, the hi-face value W+ the mean of the cell
+    and its -1 neighbour (the reference's assignment, which differs from
+    CDFACESLOPES's)."""
+    lim = mfmhd.LIM_CD
+    basis = {(1, 0, 0): (0.0, 0.5), (0, 1, 0): (0.5, 0.5), (0, 0, 1): (0.5, 0.0)}
+    for e, want in basis.items():
+        assert oracle_mod.limiter(lim, *map(float, e)) == want
+    rng = np.random.default_rng(43)
+    for _ in range(100):
+        x, y = rng.uniform(-3, 3, 3), rng.uniform(-3, 3, 3)
+        a, b = rng.uniform(-2, 2, 2)
+        lhs = oracle_mod.limiter(lim, *(a * x + b * y))
+        fx, fy = oracle_mod.limiter(lim, *x), oracle_mod.limiter(lim, *y)
+        for i in range(2):
+            assert lhs[i] == pytest.approx(a * fx[i] + b * fy[i], abs=1e-13)
+
+
 def test_limiters_differ_where_it_matters(oracle_mod):
     """The textbook forms above really separate the limiters: on a
     non-linear monotone triple no two of them give the same face value."""
