@@ -1098,7 +1098,10 @@ void helio_bflux(const Eq& E, bool radial, double rho_in, double vn_in,
                  double Bt1_in, double Bt2_in, double rho_out, double vn_out,
                  double vt1_out, double vt2_out, double p_out, double Bn_out,
                  double Bt1_out, double Bt2_out, double DX, double DY,
-                 double DZ, double* FL, double* BnValue) {
+                 double DZ, double* FL, double* BnValue,
+                 double* wchar = nullptr) {
+  // wchar (tests only): the characteristic boundary state of the sub-fast
+  // outflow branch, (rho, vn, vt1, vt2, p, Bn, Bt1, Bt2), when it is taken
   const double gamma = E.gamma, hgamma = E.hgamma;
   double U = vn_in, V = vt1_in, W = vt2_in, R = rho_in;
   double BX = Bn_in, BY = Bt1_in, BZ = Bt2_in, PG = p_in;
@@ -1204,6 +1207,10 @@ void helio_bflux(const Eq& E, bool radial, double rho_in, double vn_in,
     double E16 = PG16 / hgamma +
                  half * (R16 * ((U16 * U16) + (V16 * V16) + (W16 * W16)) + BB2);
     double BXs = d_1_4PI * BX16;
+    if (wchar) {
+      const double ws[8] = {R16, U16, V16, W16, PG16, BX16, BY16, BZ16};
+      for (int c = 0; c < 8; ++c) wchar[c] = ws[c];
+    }
     *BnValue = BX16;
     FL[0] = R16 * U16;
     FL[1] = R16 * U16 * U16 + P16 - BXs * BX16;
@@ -3174,6 +3181,21 @@ void oracle_flux_vanleer(const mfm_params* p, const double* wl,
                          const double* wr, int idir, double* out) {
   Eq E(p, 1.0, 1.0);
   neutral_flux_face(E, wl, wr, idir, 0, out);
+}
+void oracle_helio_bflux(const mfm_params* p, int radial, const double* win,
+                        const double* wout, const double* dxyz, double* out) {
+  Eq E(p, 1.0, 1.0);
+  helio_bflux(E, radial != 0, win[0], win[1], win[2], win[3], win[4], win[5],
+              win[6], win[7], wout[0], wout[1], wout[2], wout[3], wout[4],
+              wout[5], wout[6], wout[7], dxyz[0], dxyz[1], dxyz[2], out, &out[8],
+              &out[10]);
+  out[9] = std::isnan(out[10]) ? 0.0 : 1.0;
+}
+void oracle_bfluxgas(const mfm_params* p, const double* win,
+                     const double* wout, double* f5) {
+  Eq E(p, 1.0, 1.0);
+  swlism_bfluxgas(E, win[0], win[1], win[2], win[3], win[4], wout[0], wout[1],
+                  wout[2], wout[3], wout[4], f5);
 }
 void oracle_limiter(int lim, double wm, double w0, double wp, double* out) {
   limit(lim, wm, w0, wp, &out[0], &out[1]);

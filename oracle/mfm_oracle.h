@@ -78,6 +78,19 @@ void oracle_flux_face(const mfm_params* p, double ch, const double* wl,
                       const double* wr, int idir, double* out);
 void oracle_flux_vanleer(const mfm_params* p, const double* wl,
                          const double* wr, int idir, double* out);
+/* BFLUX / BFLUXR (swlism/HeliosphericF.ChF:1363-1774) in face coordinates:
+   win, wout = (rho, vn, vt1, vt2, p, Bn, Bt1, Bt2) inside and outside the
+   boundary, dxyz the outward direction (BFLUXR only); out = 8 fluxes,
+   out[8] = the face's normal field, out[9] = 1 when the sub-fast outflow
+   branch formed a characteristic boundary state, then in out[10..17]
+   (out[10..17] must hold NaN on entry) */
+void oracle_helio_bflux(const mfm_params* p, int radial, const double* win,
+                        const double* wout, const double* dxyz, double* out);
+/* BFLUXGAS (swlism/SWLISMF.ChF:1434-1553): win, wout = (rho, vn, vt1, vt2,
+   p) of a neutral population; f5 holds the Riemann flux on entry and the
+   boundary flux on return */
+void oracle_bfluxgas(const mfm_params* p, const double* win,
+                     const double* wout, double* f5);
 /* 1-D face values of cell 0 from (w_-1, w_0, w_+1): out[0]=W-, out[1]=W+ */
 void oracle_limiter(int lim, double wm, double w0, double wp, double* out);
 /* cons -> prim for one cell (CONSTOPRIM / CONSTOPRIM_MF) */

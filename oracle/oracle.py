@@ -54,6 +54,8 @@ def load():
     lib.oracle_flux_dedner.argtypes = [P, C.c_double, dp, dp, C.c_int, C.c_int, dp]
     lib.oracle_flux_vanleer.argtypes = [P, dp, dp, C.c_int, dp]
     lib.oracle_flux_face.argtypes = [P, C.c_double, dp, dp, C.c_int, dp]
+    lib.oracle_helio_bflux.argtypes = [P, C.c_int, dp, dp, dp, dp]
+    lib.oracle_bfluxgas.argtypes = [P, dp, dp, dp]
     lib.oracle_limiter.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp]
     lib.oracle_face_slopes.argtypes = [C.c_int, C.c_double, C.c_double, dp]
     lib.oracle_face_values.argtypes = [P, mfmhd.Fab, C.c_int, mfmhd.Box, mfmhd.Fab,
@@ -201,6 +203,22 @@ def flux_vanleer(p, wl, wr, idir):
     load().oracle_flux_vanleer(C.byref(p), _ptr(_vec(wl)), _ptr(_vec(wr)), idir,
                                _ptr(out))
     return out[:5]
+
+
+def helio_bflux(p, win, wout, radial=False, dxyz=(0.0, 0.0, 0.0)):
+    """BFLUX (radial: BFLUXR) in face coordinates: (8 fluxes, face Bn, the
+    characteristic boundary state of the sub-fast outflow branch or None)."""
+    out = np.full(18, np.nan)
+    load().oracle_helio_bflux(C.byref(p), int(radial), _ptr(_vec(win)), _ptr(_vec(wout)),
+                              _ptr(_vec(dxyz)), _ptr(out))
+    return out[:8], out[8], (out[10:18].copy() if out[9] == 1.0 else None)
+
+
+def bfluxgas(p, win, wout, riemann):
+    """BFLUXGAS of one neutral population; riemann = the face's Riemann flux."""
+    f5 = np.array(riemann, dtype=np.float64)
+    load().oracle_bfluxgas(C.byref(p), _ptr(_vec(win)), _ptr(_vec(wout)), _ptr(f5))
+    return f5
 
 
 def limiter(lim, wm, w0, wp):

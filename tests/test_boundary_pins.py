@@ -1,0 +1,196 @@
+"""Pins of the oracle's characteristic boundary fluxes of the heliospheric and
+SW/LISM problems -- BFLUX / BFLUXR (swlism/HeliosphericF.ChF:1363-1774) for
+the plasma and BFLUXGAS (swlism/SWLISMF.ChF:1434-1553) for a neutral
+population -- against what gas dynamics and MHD fix for them:
+
+- supersonic (super-fast) out- or inflow: the physical flux of the upwind
+  state;
+- BFLUXGAS's sub-sonic branches: the sonic state reached along the
+  characteristic that leaves the domain (out) or enters it (in), i.e. the
+  Riemann invariant u +- 2a/(gamma - 1) and the entropy p / rho^gamma of
+  the state it starts from, with u = +-a: solved here by a root finder
+  from those three conditions, not from the reference's closed form;
+- BFLUX's sub-fast outflow branch: a state moved from the interior along
+  the right eigenvector of the u - c_f wave (from the ideal-MHD Jacobian
+  written from the equations, Gaussian units) until it is fast-sonic,
+  linearised in rho: the state agrees with W + h r to first order in
+  c_f - u (h from the linearised sonic condition), its u - c_f vanishes
+  to second order, and the flux is that state's physical flux;
+- the sub-fast inflow branch: the 8-wave Roe flux of (interior, exterior).
+
+CPU only."""
+import numpy as np
+import pytest
+from scipy.optimize import fsolve
+
+import mfmhd
+from test_oracle_closed_forms import PI4, mhd_flux, prim_jacobian
+from test_oracle_pins import euler_flux
+
+G = 5.0 / 3.0
+
+
+def plasma_params():
+    p = mfmhd.orszag_params(G)
+    p.flux = mfmhd.FLUX_ROE8
+    return p
+
+
+def fast_speed(w):
+    rho, _, _, _, pr, bx, by, bz = w
+    a2 = G * pr / rho
+    b2 = (bx * bx + by * by + bz * bz) / (PI4 * rho)
+    return np.sqrt(0.5 * (a2 + b2 + np.sqrt((a2 + b2) ** 2 - 4 * a2 * bx * bx / (PI4 * rho))))
+
+
+STATES = [  # (rho, vn, vt1, vt2, p, Bn, Bt1, Bt2); vn is set per case
+    np.array([1.3, 0.0, 0.2, -0.1, 0.9, 1.1, 0.8, -0.5]),
+    np.array([0.7, 0.0, 0.3, 0.0, 1.6, -2.2, 0.4, 1.3]),
+    np.array([2.1, 0.0, 0.1, -0.3, 0.4, 0.6, -1.7, 0.2]),
+]
+
+
+@pytest.mark.parametrize("radial", [False, True], ids=["BFLUX", "BFLUXR"])
+@pytest.mark.parametrize("k", range(len(STATES)))
+def test_bflux_super_fast(oracle_mod, radial, k):
+    """u >= c_f: the interior state's flux and normal field; u < -c_f: the
+    exterior state's."""
+    p = plasma_params()
+    w = STATES[k].copy()
+    out = STATES[(k + 1) % 3].copy()
+    w[1] = 1.2 * fast_speed(w)
+    f, bn, ws = oracle_mod.helio_bflux(p, w, out, radial, (1.0, 0.2, 0.1))
+    np.testing.assert_allclose(f, mhd_flux(w, G), rtol=1e-14, atol=1e-14)
+    assert bn == w[5] and ws is None
+    w[1] = -1.2 * fast_speed(w)
+    f, bn, ws = oracle_mod.helio_bflux(p, w, out, radial, (1.0, 0.2, 0.1))
+    np.testing.assert_allclose(f, mhd_flux(out, G), rtol=1e-14, atol=1e-14)
+    assert bn == out[5] and ws is None
+
+
+@pytest.mark.parametrize("radial", [False, True], ids=["BFLUX", "BFLUXR"])
+@pytest.mark.parametrize("k", range(len(STATES)))
+def test_bflux_sub_fast_outflow_is_fast_sonic(oracle_mod, radial, k):
+    """0 <= u = c_f (1 - delta): the boundary state is W + h r + O(delta^2)
+    with r the u - c_f eigenvector (normalised to d rho = rho) and h the root
+    of the linearised condition u(W + h r) = c_f(W + h r); the residual
+    u - c_f of the state is O(delta^2); the flux is its physical flux and
+    the face's normal field its Bn.  Checked at delta = 1e-2, 1e-3, 1e-4:
+    the state error falls tenfold and the residual a hundredfold per step
+    (a wrong term in the linearisation leaves an O(delta) residual)."""
+    p = plasma_params()
+    errs, resid = [], []
+    for delta in (1e-2, 1e-3, 1e-4):
+        w = STATES[k].copy()
+        w[1] = fast_speed(w) * (1 - delta)
+        out = w.copy()
+        out[1] = 5.0          # exterior leaving faster than its c_f (BFLUX's condition)
+        lam, R = np.linalg.eig(prim_jacobian(w, G))
+        lam, R = lam.real, R.real
+        j = np.argmin(lam)
+        assert lam[j] == pytest.approx(w[1] - fast_speed(w), rel=1e-10)
+        r = R[:, j] / R[0, j] * w[0]
+        e = 1e-7
+        dcf = (fast_speed(w + e * r) - fast_speed(w - e * r)) / (2 * e)
+        h = (fast_speed(w) - w[1]) / (r[1] - dcf)
+        f, bn, ws = oracle_mod.helio_bflux(p, w, out, radial, (1.0, 0.2, 0.1))
+        assert ws is not None
+        np.testing.assert_allclose(f, mhd_flux(ws, G), rtol=1e-13, atol=1e-14)
+        assert bn == ws[5] == w[5]
+        errs.append(np.max(np.abs((ws - w) - h * r)) / np.max(np.abs(h * r)))
+        resid.append(abs(ws[1] - fast_speed(ws)) / fast_speed(w))
+    for a, b in zip(errs, errs[1:]):
+        assert b < 0.15 * a
+    for a, b in zip(resid, resid[1:]):
+        assert b < 0.015 * a
+    assert errs[-1] < 1e-3 and resid[-1] < 1e-8
+
+
+@pytest.mark.parametrize("radial", [False, True], ids=["BFLUX", "BFLUXR"])
+def test_bflux_sub_fast_inflow_is_roe(oracle_mod, radial):
+    """-c_f < u < 0: the 8-wave Roe flux of (interior, exterior) and the mean
+    normal field; BFLUX also takes it for a sub-fast outflow whose exterior
+    is not super-fast."""
+    p = plasma_params()
+    rng = np.random.default_rng(61)
+    for t in range(20):
+        w = STATES[t % 3].copy()
+        out = STATES[(t + 1) % 3].copy() * rng.uniform(0.8, 1.2, 8)
+        w[1] = -rng.uniform(0.05, 0.95) * fast_speed(w)
+        out[1] = rng.uniform(-0.5, 0.5)
+        f, bn, ws = oracle_mod.helio_bflux(p, w, out, radial, (1.0, 0.2, 0.1))
+        assert ws is None
+        np.testing.assert_allclose(f, oracle_mod.flux_roe8(p, w, out, 0), rtol=1e-14, atol=1e-14)
+        assert bn == pytest.approx(0.5 * (w[5] + out[5]), rel=1e-15)
+        if not radial:
+            w[1] = -w[1]                       # sub-fast outflow
+            out[1] = 0.5 * fast_speed(out)     # exterior not super-fast
+            f, bn, ws = oracle_mod.helio_bflux(p, w, out, radial)
+            assert ws is None
+            np.testing.assert_allclose(f, oracle_mod.flux_roe8(p, w, out, 0),
+                                       rtol=1e-14, atol=1e-14)
+
+
+# ---------------------------------------------------------------- BFLUXGAS --
+def gas_params():
+    p = mfmhd.orszag_params(G)
+    p.smallr = 1e-12
+    p.smallp = 1e-12
+    return p
+
+
+def sonic_state(w, sign):
+    """The state with u = sign * a on the characteristic of w carrying
+    u + sign 2a/(gamma-1), at w's entropy; tangential velocity kept."""
+    rho, u, v, ww, pr = w
+    J = u + sign * 2 * np.sqrt(G * pr / rho) / (G - 1)
+    s = pr / rho ** G
+
+    def eqs(x):
+        r, un, pg = np.exp(x[0]), x[1], np.exp(x[2])
+        a = np.sqrt(G * pg / r)
+        return [un + sign * 2 * a / (G - 1) - J, np.log(pg / r ** G) - np.log(s), un - sign * a]
+    x = fsolve(eqs, [np.log(rho), sign * np.sqrt(G * pr / rho), np.log(pr)], xtol=1e-12)
+    assert np.max(np.abs(eqs(x))) < 1e-11
+    return np.array([np.exp(x[0]), x[1], v, ww, np.exp(x[2])])
+
+
+GAS = [np.array([1.0, 0.0, 0.3, -0.2, 0.6]), np.array([0.3, 0.0, -0.1, 0.4, 0.05]),
+       np.array([2.5, 0.0, 0.0, 0.0, 3.0])]
+
+
+@pytest.mark.parametrize("k", range(len(GAS)))
+def test_bfluxgas_branches(oracle_mod, k):
+    p = gas_params()
+    w = GAS[k].copy()
+    a = np.sqrt(G * w[4] / w[0])
+    ext = GAS[(k + 1) % 3].copy()
+    a1 = np.sqrt(G * ext[4] / ext[0])
+    riemann = np.array([0.11, 0.22, 0.33, 0.44, 0.55])
+    # supersonic outflow: the interior's flux; supersonic inflow: the exterior's
+    w[1], ext[1] = 1.3 * a, 0.0
+    np.testing.assert_allclose(oracle_mod.bfluxgas(p, w, ext, riemann), euler_flux(G, w, 0),
+                               rtol=1e-14, atol=1e-15)
+    w[1], ext[1] = -1.3 * a, -0.2
+    np.testing.assert_allclose(oracle_mod.bfluxgas(p, w, ext, riemann), euler_flux(G, ext, 0),
+                               rtol=1e-14, atol=1e-15)
+    # sub-sonic outflow: the sonic state on the outgoing characteristic
+    # (J+ and the entropy of the interior) when the exterior leaves
+    # supersonically, else the Riemann flux is kept
+    for frac in (0.0, 0.4, 0.9):
+        w[1], ext[1] = frac * a, 1.5 * a1
+        s = sonic_state(w, +1)
+        np.testing.assert_allclose(oracle_mod.bfluxgas(p, w, ext, riemann), euler_flux(G, s, 0),
+                                   rtol=1e-12, atol=1e-14)
+        ext[1] = 0.5 * a1
+        assert np.array_equal(oracle_mod.bfluxgas(p, w, ext, riemann), riemann)
+    # sub-sonic inflow: the sonic state on the entering characteristic (J-
+    # and the entropy of the exterior) when the exterior arrives
+    # supersonically, else the Riemann flux is kept
+    for frac in (0.1, 0.5, 0.95):
+        w[1], ext[1] = -frac * a, -1.5 * a1
+        s = sonic_state(ext, -1)
+        np.testing.assert_allclose(oracle_mod.bfluxgas(p, w, ext, riemann), euler_flux(G, s, 0),
+                                   rtol=1e-12, atol=1e-14)
+        ext[1] = -0.5 * a1
+        assert np.array_equal(oracle_mod.bfluxgas(p, w, ext, riemann), riemann)

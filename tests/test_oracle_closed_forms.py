@@ -423,6 +423,35 @@ def test_charge_exchange_cold_beam_limit(oracle_mod):
         assert np.isclose(S[12] / dt, -0.5 * rate * (uh @ uh), rtol=1e-6, atol=0)
 
 
+@pytest.mark.parametrize("single", [0, 1])
+def test_charge_exchange_cross_section_is_lindsay_stebbings(oracle_mod, single):
+    """The cross section is Lindsay & Stebbings' fit (J. Geophys. Res. 110,
+    A12213, 2005), sigma = (4.15 - 0.531 ln E)^2 (1 - exp(-67.3 / E))^4.5
+    x 1e-16 cm^2, in the reference's variable E = U / U_1 with U_1 the
+    speed of a 1 keV proton, sqrt(2 keV / m_p) = 4.377e7 cm/s: its
+    constants 13.493 = 4.15 + 0.531 ln U_1 and 2.946e9 = 67.3 U_1 are the
+    published ones carried through that substitution, to their printed
+    digits.  Read from cold beams (the rate is sigma rho rho_H |du|, the
+    neutral's mass loss in a region where it is not born) over U from
+    1e5 to 3e9 cm/s."""
+    keV, m_p = 1.602176634e-9, 1.67262192e-24   # erg, g
+    U1 = np.sqrt(2 * keV / m_p)
+    p = ce_params(2)
+    p.ce_single_literals = single
+    rho, rh, dt = 0.7, 0.2, 1e-3
+    w = ce_state(rho, (1.0, 0.0, 0.0), 1e-14, [(rh, (0.0, 0.0, 0.0), 1e-14)])
+    for U in 10 ** np.linspace(5, np.log10(3e9), 25):
+        p.ce_scale_vel = U   # |du| = 1 in units of ce_scale_vel
+        rate = -oracle_mod.charge_exchange(p, w, 2, dt)[8] / dt
+        sigma = rate / (rho * rh * 1.0) / p.ce_sigma_lindsay   # units of 1e-16 cm^2
+        E = U / U1
+        t, x = 4.15 - 0.531 * np.log(E), 67.3 / E
+        want = t ** 2 * (1 - np.exp(-x)) ** 4.5
+        # half a unit in the last printed digit of 13.493 and of 2.946e9
+        rel = 2 * 5e-4 / t + 4.5 * x * np.exp(-x) / (1 - np.exp(-x)) * 0.5e-3 / 2.946 + 1e-6
+        assert abs(sigma - want) <= rel * want, (U, sigma, want)
+
+
 def test_charge_exchange_mean_relative_speed(oracle_mod):
     """The exchange rate is sigma(U*) rho rho_H U*, and with no drift U* is
     the mean relative speed of two Maxwellians, sqrt(4/pi (v_p^2 + v_H^2))
