@@ -8,6 +8,14 @@
 // in the same operation order as the ChomboFortran kernels, so that it can be
 // compared against both the reference's semantics and the fused HIP kernels.
 // Paths are relative to MHDAM-3.2.4/ of the reference.
+//
+// Parity unpinned (no closed form and no fixture of the reference checks
+// them; the device matches this file on them, so a mistake common to both
+// would pass): in helio_bc_face the neutral branches of HELIOBC; in helio_gs
+// the ghost copies of neutral populations 2 and 3 (HELIOGS), and the same
+// copies in the SW/LISM x pass; in swlism_reinit the floor states of
+// populations 2 and 3 (SWLISMREINIT).  Everything else is pinned by the
+// closed-form, invariant and reference-fixture tests beside it.
 #include "mfm_oracle.h"
 
 #include <algorithm>
@@ -3190,6 +3198,12 @@ void oracle_helio_bflux(const mfm_params* p, int radial, const double* win,
               wout[5], wout[6], wout[7], dxyz[0], dxyz[1], dxyz[2], out, &out[8],
               &out[10]);
   out[9] = std::isnan(out[10]) ? 0.0 : 1.0;
+}
+void oracle_helio_bc_face(const mfm_params* p, int lohisign, int dir, double dx,
+                          const int* ijk, const double* wleft,
+                          const double* wright, double* f, double* bn) {
+  Eq E(p, 1.0, 1.0);
+  helio_bc_face(E, lohisign, dir, dx, ijk[0], ijk[1], ijk[2], wleft, wright, f, bn);
 }
 void oracle_bfluxgas(const mfm_params* p, const double* win,
                      const double* wout, double* f5) {

@@ -86,6 +86,13 @@ void oracle_flux_vanleer(const mfm_params* p, const double* wl,
    (out[10..17] must hold NaN on entry) */
 void oracle_helio_bflux(const mfm_params* p, int radial, const double* win,
                         const double* wout, const double* dxyz, double* out);
+/* HELIOBC (swlism/HeliosphericF.ChF:221-925) at one physical boundary face
+   of cell (ijk) in direction dir on side lohisign: wleft = W+ of the cell
+   below the face, wright = W- of the cell above (full primitive vectors);
+   f = the face's flux vector, bn = its normal field */
+void oracle_helio_bc_face(const mfm_params* p, int lohisign, int dir, double dx,
+                          const int* ijk, const double* wleft,
+                          const double* wright, double* f, double* bn);
 /* BFLUXGAS (swlism/SWLISMF.ChF:1434-1553): win, wout = (rho, vn, vt1, vt2,
    p) of a neutral population; f5 holds the Riemann flux on entry and the
    boundary flux on return */

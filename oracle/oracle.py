@@ -56,6 +56,7 @@ def load():
     lib.oracle_flux_face.argtypes = [P, C.c_double, dp, dp, C.c_int, dp]
     lib.oracle_helio_bflux.argtypes = [P, C.c_int, dp, dp, dp, dp]
     lib.oracle_bfluxgas.argtypes = [P, dp, dp, dp]
+    lib.oracle_helio_bc_face.argtypes = [P, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int), dp, dp, dp, dp]
     lib.oracle_limiter.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp]
     lib.oracle_face_slopes.argtypes = [C.c_int, C.c_double, C.c_double, dp]
     lib.oracle_face_values.argtypes = [P, mfmhd.Fab, C.c_int, mfmhd.Box, mfmhd.Fab,
@@ -212,6 +213,16 @@ def helio_bflux(p, win, wout, radial=False, dxyz=(0.0, 0.0, 0.0)):
     load().oracle_helio_bflux(C.byref(p), int(radial), _ptr(_vec(win)), _ptr(_vec(wout)),
                               _ptr(_vec(dxyz)), _ptr(out))
     return out[:8], out[8], (out[10:18].copy() if out[9] == 1.0 else None)
+
+
+def helio_bc_face(p, lohisign, d, dx, ijk, wleft, wright):
+    """HELIOBC at one physical boundary face: (flux vector, normal field)."""
+    nc = mfmhd.num_components(p)
+    f = np.zeros(32)
+    bn = np.zeros(1)
+    load().oracle_helio_bc_face(C.byref(p), lohisign, d, dx, (C.c_int * 3)(*ijk),
+                                _ptr(_vec(wleft)), _ptr(_vec(wright)), _ptr(f), _ptr(bn))
+    return f[:nc], bn[0]
 
 
 def bfluxgas(p, win, wout, riemann):

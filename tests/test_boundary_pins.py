@@ -194,3 +194,112 @@ def test_bfluxgas_branches(oracle_mod, k):
                                    rtol=1e-12, atol=1e-14)
         ext[1] = -0.5 * a1
         assert np.array_equal(oracle_mod.bfluxgas(p, w, ext, riemann), riemann)
+
+
+# ------------------------------------------------- HELIOBC, neutral fluids --
+def helio4():
+    p = mfmhd.helio_params(lismN=0.06, lismV=2.6e6, sunN=7.0, sunV=4.5e7, netN=0.1,
+                           R0=0.1, dx=1.0 / 64, num_cells=(64, 64, 64), sunXYZ=(0.5, 0.5, 0.5))
+    p.nfluids = 4
+    return p
+
+
+# the floors HELIOBC applies to neutral populations 2 and 3 (rho, p)
+FLOORS = {2: (1.0e-6, 1.0e-8), 3: (1.0e-7, 1.0e-9)}
+
+
+def neutral_slice(f):
+    b = 8 + 5 * (f - 1)
+    return slice(b, b + 5)
+
+
+def lab_euler(w5, d):
+    return euler_flux(G, w5, d)
+
+
+def helio_face(oracle_mod, p, side, d, interior, exterior):
+    """HELIOBC's flux at the face of direction d on side +-1 with the
+    interior cell's face state and the exterior's."""
+    wl, wr = (interior, exterior) if side == 1 else (exterior, interior)
+    ijk = [32, 32, 32]
+    ijk[d] = 63 if side == 1 else 0
+    return oracle_mod.helio_bc_face(p, side, d, 1.0 / 64, ijk, wl, wr)[0]
+
+
+def two_states(rng):
+    from test_oracle_pins import rand_prim
+    a, b = rand_prim(rng, nf=4), rand_prim(rng, nf=4)
+    return a, b
+
+
+@pytest.mark.parametrize("f", [2, 3])
+def test_helio_bc_populations_2_3_outflow_only(oracle_mod, f):
+    """Populations 2 and 3 leave through every boundary face with the
+    physical flux of the interior's floored state, the normal velocity
+    clamped to leave (none enters); the exterior state plays no part."""
+    p = helio4()
+    rng = np.random.default_rng(70 + f)
+    sl = neutral_slice(f)
+    dmin, pmin = FLOORS[f]
+    for t in range(36):
+        d, side = t % 3, (1 if (t // 3) % 2 else -1)
+        w_in, w_out = two_states(rng)
+        if t % 4 == 0:   # below the floors
+            w_in[sl.start], w_in[sl.start + 4] = 0.1 * dmin, 0.1 * pmin
+        got = helio_face(oracle_mod, p, side, d, w_in, w_out)[sl]
+        s = w_in[sl].copy()
+        s[0], s[4] = max(dmin, s[0]), max(pmin, s[4])
+        s[1 + d] = max(0.0, s[1 + d]) if side == 1 else min(0.0, s[1 + d])
+        np.testing.assert_allclose(got, lab_euler(s, d), rtol=1e-14, atol=1e-300)
+        w_out2 = w_out.copy()
+        w_out2[sl] *= 1.7
+        assert np.array_equal(helio_face(oracle_mod, p, side, d, w_in, w_out2)[sl], got)
+
+
+def test_helio_bc_population_1_low_x_outflow(oracle_mod):
+    """At the low-x face (the downwind boundary of the LISM flow along -x)
+    population 1 leaves with the interior state's physical flux, the
+    normal velocity clamped to leave (at least 1e-6)."""
+    p = helio4()
+    rng = np.random.default_rng(81)
+    sl = neutral_slice(1)
+    for _ in range(20):
+        w_in, w_out = two_states(rng)
+        got = helio_face(oracle_mod, p, -1, 0, w_in, w_out)[sl]
+        s = w_in[sl].copy()
+        s[1] = min(-1e-6, s[1])
+        np.testing.assert_allclose(got, lab_euler(s, 0), rtol=1e-14, atol=1e-300)
+
+
+def test_helio_bc_population_1_van_leer(oracle_mod):
+    """At the other boundary faces population 1 takes van Leer's flux of
+    (below, above) in the lab frame; when the interior state leaves, the
+    exterior takes its velocity and density, so that an
+    interior leaving at the exterior's pressure has its own physical flux
+    (consistency) -- on the low side as the point image of the high side."""
+    p = helio4()
+    rng = np.random.default_rng(91)
+    sl = neutral_slice(1)
+    n = 0
+    for t in range(60):
+        d, side = t % 3, (1 if (t // 3) % 2 else -1)
+        if side == -1 and d == 0:
+            continue
+        w_in, w_out = two_states(rng)
+        got = helio_face(oracle_mod, p, side, d, w_in, w_out)[sl]
+        a, b = w_in[sl].copy(), w_out[sl].copy()
+        if side * a[1 + d] > 0:          # the interior leaves
+            b[0:4] = a[0:4]
+        lo, hi = (a, b) if side == 1 else (b, a)
+        want = oracle_mod.flux_vanleer(p, lo, hi, d)
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-14 * np.max(np.abs(want)))
+        # consistency: leaving at the exterior's pressure (and density)
+        w_out3 = w_out.copy()
+        w_out3[sl.start + 4] = w_in[sl.start + 4]
+        w_out3[sl.start] = w_in[sl.start]
+        w_in3 = w_in.copy()
+        w_in3[sl.start + 1 + d] = side * abs(w_in[sl.start + 1 + d])
+        got3 = helio_face(oracle_mod, p, side, d, w_in3, w_out3)[sl]
+        np.testing.assert_allclose(got3, lab_euler(w_in3[sl], d), rtol=1e-12, atol=1e-14)
+        n += 1
+    assert n >= 40
