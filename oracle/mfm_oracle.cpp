@@ -11,10 +11,9 @@
 //
 // Parity unpinned (no closed form and no fixture of the reference checks
 // them; the device matches this file on them, so a mistake common to both
-// would pass): in helio_bc_face the neutral branches of HELIOBC; in helio_gs
-// the ghost copies of neutral populations 2 and 3 (HELIOGS), and the same
-// copies in the SW/LISM x pass; in swlism_reinit the floor states of
-// populations 2 and 3 (SWLISMREINIT).  Everything else is pinned by the
+// would pass): in helio_gs the ghost copies of neutral populations 2 and 3
+// (HELIOGS), and the same copies in the SW/LISM x pass; in swlism_reinit the
+// floor states of populations 2 and 3 (SWLISMREINIT).  Everything else is pinned by the
 // closed-form, invariant and reference-fixture tests beside it.
 #include "mfm_oracle.h"
 
