@@ -253,7 +253,7 @@ int mfm_ctx_get_timing(mfm_ctx* ctx, int stage, int kind, double* ms,
  * fluxes, then k_update.  MFM_STAGE_SUMMED: after k_prim and the
  * boundary-face kernel, one tile kernel per fluid group (the plasma with
  * psi and the normal B, then each neutral fluid) solves the interior faces
- * of a 64 x 4 column tile, z-chunks of tz planes (default 16), into LDS and
+ * of a 64 x 8 column tile, z-chunks of tz planes (default 16), into LDS and
  * writes per cell the face-flux differences summed over the directions;
  * k_update applies those sums (FP32-flux build only; the FP64 build keeps
  * the reference's order, the directions one at a time, and runs the
@@ -261,8 +261,8 @@ int mfm_ctx_get_timing(mfm_ctx* ctx, int stage, int kind, double* ms,
  * solves every face and updates the cells; the face fluxes never reach HBM,
  * but its register budget makes it slower on the 256^3 workload.
  * MFM_STAGE_AUTO (default): per batch, the summed stages in the FP32-flux
- * build when the batch's cells fill at least 3/4 of their 64 x 4 column
- * tiles (12.4 against 13.9-14.2 ms per step on the 256^3 workload), else the
+ * build when the batch's cells fill at least 3/4 of their 64 x 8 column
+ * tiles (12.0 against 13.9-14.2 ms per step on the 256^3 workload), else the
  * separate kernels; the allocation-free entry point always takes the
  * separate kernels.  AUTO decides per call: in the FP32-flux build a patch
  * updated alone and in a level can then take different structures (results

@@ -100,7 +100,7 @@ struct RegionBuilder {
     v.push_back(r);
   }
   // a region of a tile kernel: one block per tx x ty column tile and
-  // z-chunk of tz planes (k_stage: 16 x 16, k_fsum: 64 x 4)
+  // z-chunk of tz planes (k_stage: 16 x 16, k_fsum: 64 x 8)
   void add_tiles(int patch, const HBox& b, int tz, int tx = 16, int ty = 16) {
     long cells = b.size();
     if (cells <= 0) return;
@@ -341,7 +341,7 @@ int run_batch(mfm_ctx* c, const Consts* Ct, int nct, const PatchIn* in, int n,
   // applies the directions one at a time, in the reference's order
   int mode = (han || C0.char_rec) ? MFM_STAGE_SEPARATE : c->stage_mode;
   if (mode == MFM_STAGE_AUTO) {
-    // the summed stages where their 64 x 4 column tiles are well filled
+    // the summed stages where their 64 x 8 column tiles are well filled
     // (FP32-flux build; the allocation-free entry point keeps the separate
     // kernels, whose scratch its bound is stated for)
     mode = MFM_STAGE_SEPARATE;

@@ -115,8 +115,12 @@ hipError_t launch_stage_fused(const Consts& C, const Consts* CT, const DevPatch*
                               long* idx_part, hipStream_t s, KernelTimer* T);
 
 // Column tile of the face-sum kernels (k_fsum): long in x, so that the
-// faces a wavefront solves lie along contiguous rows.
-constexpr int kSumTX = 64, kSumTY = 4;
+// faces a wavefront solves lie along contiguous rows; one thread per column.
+// A tile solves (TX+1)·TY + TX·(TY+1) + TX·TY faces per plane in whole
+// wavefront passes: 14 passes per 256 columns for 64 x 4, 26 per 512 for
+// 64 x 8 (7 % fewer per cell; 11.99 against 12.41 ms per 256^3 step), while
+// 64 x 16 (50 per 1024, one block per CU) is slower again (12.31 ms).
+constexpr int kSumTX = 64, kSumTY = 8, kSumBlock = kSumTX * kSumTY;
 
 // Summed stages: the face-sum kernels over L.sum (after launch_stage has run
 // k_prim and the boundary-face fluxes), then k_update reading their sums.

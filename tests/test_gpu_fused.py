@@ -241,7 +241,7 @@ def test_summed_fp32_helio_and_brio_wu(oracle_mod, nf):
 
 def test_auto_mode_choice(oracle_mod):
     """MFM_STAGE_AUTO, the default: the summed stages where the cells fill
-    at least 3/4 of their 64 x 4 column tiles in the FP32-flux build, the
+    at least 3/4 of their 64 x 8 column tiles in the FP32-flux build, the
     separate kernels otherwise (a 20-wide box, the FP64 build, the
     allocation-free entry point); each choice meets its bar against the
     oracle."""
