@@ -3119,6 +3119,16 @@ int oracle_init_problem(const mfm_params* p, mfm_fab Uf, mfm_box bx, double dx) 
   return MFM_OK;
 }
 
+// The problem's physical-boundary ghost fill of direction dir on a primitive
+// FAB (PhysBC::fillGhostCells of the problem), dx the cell size.
+int oracle_fill_ghost(const mfm_params* p, mfm_fab Wf, int dir, double dx) {
+  int rc = check_args(p);
+  if (rc) return rc;
+  Eq E(p, 1.0, 1.0);
+  Fab W; W.wrap(Wf);
+  fill_ghost(E, W, dir, dx);
+  return MFM_OK;
+}
 int oracle_fill_periodic(const mfm_params* p, mfm_fab Uf, mfm_box validb) {
   Fab U; U.wrap(Uf);
   Box v = mkbox(validb);

@@ -78,6 +78,10 @@ void oracle_flux_face(const mfm_params* p, double ch, const double* wl,
                       const double* wr, int idir, double* out);
 void oracle_flux_vanleer(const mfm_params* p, const double* wl,
                          const double* wr, int idir, double* out);
+/* the problem's physical-boundary ghost fill of direction dir on a
+   primitive FAB (HELIOGS, LISMINIT and the axis reflection, the Riemann
+   tube's copies); dx the cell size */
+int oracle_fill_ghost(const mfm_params* p, mfm_fab W, int dir, double dx);
 /* BFLUX / BFLUXR (swlism/HeliosphericF.ChF:1363-1774) in face coordinates:
    win, wout = (rho, vn, vt1, vt2, p, Bn, Bt1, Bt2) inside and outside the
    boundary, dxyz the outward direction (BFLUXR only); out = 8 fluxes,

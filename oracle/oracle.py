@@ -49,6 +49,7 @@ def load():
     lib.oracle_compute_dt.argtypes = [P, mfmhd.Fab, mfmhd.Box, C.c_double, dp, ip]
     lib.oracle_init_problem.argtypes = [P, mfmhd.Fab, mfmhd.Box, C.c_double]
     lib.oracle_fill_periodic.argtypes = [P, mfmhd.Fab, mfmhd.Box]
+    lib.oracle_fill_ghost.argtypes = [P, mfmhd.Fab, C.c_int, C.c_double]
     lib.oracle_flux_roe8.argtypes = [P, dp, dp, C.c_int, C.c_int, C.c_int, dp]
     lib.oracle_flux_rusanov.argtypes = [P, dp, dp, C.c_int, C.c_int, dp]
     lib.oracle_flux_dedner.argtypes = [P, C.c_double, dp, dp, C.c_int, C.c_int, dp]
@@ -162,6 +163,14 @@ def fill_periodic(p, U, lo, valid_lo, valid_hi):
                                   mfmhd.Box.of(valid_lo, valid_hi))
     assert rc == 0
     return U
+
+
+def fill_ghost(p, W, lo, d, dx):
+    """The problem's physical-boundary ghost fill of direction d on the
+    primitive FAB W (shape (nc, nz, ny, nx) at lo), in place."""
+    rc = load().oracle_fill_ghost(C.byref(p), fab_of(W, lo), d, dx)
+    assert rc == 0
+    return W
 
 
 def _vec(x, n=32):

@@ -303,3 +303,77 @@ def test_helio_bc_population_1_van_leer(oracle_mod):
         np.testing.assert_allclose(got3, lab_euler(w_in3[sl], d), rtol=1e-12, atol=1e-14)
         n += 1
     assert n >= 40
+
+
+# ------------------------------------------------------------ ghost copies --
+def _box_fab(rng, nc, lo, hi):
+    shape = (nc, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+    return rng.uniform(0.5, 2.0, shape)
+
+
+@pytest.mark.parametrize("side", [1, -1])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_heliogs_ghosts(oracle_mod, d, side):
+    """HELIOGS (swlism/HeliosphericF.ChF:978-1149) on every face of the
+    heliospheric box: the plasma ghosts hold the LISM state, neutral
+    population 1 the LISM velocity at its own density and pressure, and
+    populations 2 and 3 copy the interior cell next to the boundary in their
+    row (zero-order extrapolation); the interior is untouched."""
+    p = helio4()
+    nc = mfmhd.num_components(p)
+    rng = np.random.default_rng(100 + 3 * d + side)
+    lo, hi = [28, 28, 28], [35, 35, 35]
+    if side == 1:
+        lo[d], hi[d] = 58, 63 + 4
+    else:
+        lo[d], hi[d] = -4, 5
+    W0 = _box_fab(rng, nc, lo, hi)
+    W = oracle_mod.fill_ghost(p, W0.copy(), lo, d, 1.0 / 64)
+    ax = 3 - d                       # numpy axis of direction d
+    idx = np.arange(lo[d], hi[d] + 1)
+    ghost = idx > 63 if side == 1 else idx < 0
+    edge = (63 if side == 1 else 0) - lo[d]
+
+    def take(A, sel):
+        return np.take(A, np.nonzero(sel)[0] if sel.dtype == bool else sel, axis=ax)
+    assert np.array_equal(take(W, ~ghost), take(W0, ~ghost))
+    G = take(W, ghost)
+    lism = [1.0, p.lism_ux, p.lism_uy, p.lism_uz, p.lism_p, p.lism_bx, p.lism_by, p.lism_bz]
+    for c, v in enumerate(lism):
+        assert np.all(G[c] == v), c
+    net = [p.net_rho, p.lism_ux, p.lism_uy, p.lism_uz, p.net_p]
+    for c, v in enumerate(net):
+        assert np.all(G[8 + c] == v), c
+    E = take(W0, np.array([edge]))
+    for c in range(13, 23):
+        assert np.array_equal(G[c], np.broadcast_to(E[c], G[c].shape)), c
+
+
+@pytest.mark.parametrize("side,d", [(1, 0), (-1, 0), (1, 1)])
+def test_swlism_ghost_copies_of_populations_2_3(oracle_mod, side, d):
+    """LISMINIT's ghosts (swlism/SWLISMF.ChF:512-652) on the x boundaries
+    and the top: populations 2 and 3 copy the interior cell next to the
+    boundary in their row; the interior is untouched."""
+    nx, ny, dx = 48, 32, 1.0
+    p = mfmhd.swlism_params(sunXY=(0.4 * nx * dx, 0.0), R0=2.6 * dx, initR=6.0 * dx, dx=dx,
+                            num_cells=(nx, ny), nfluids=4)
+    nc = mfmhd.num_components(p)
+    rng = np.random.default_rng(120 + 3 * d + side)
+    lo, hi = [20, 10, 0], [27, 17, 0]
+    top = (nx - 1, ny - 1)[d]
+    if side == 1:
+        lo[d], hi[d] = top - 5, top + 4
+    else:
+        lo[d], hi[d] = -4, 5
+    W0 = _box_fab(rng, nc, lo, hi)
+    W = oracle_mod.fill_ghost(p, W0.copy(), lo, d, dx)
+    ax = 3 - d
+    idx = np.arange(lo[d], hi[d] + 1)
+    ghost = idx > top if side == 1 else idx < 0
+    edge = (top if side == 1 else 0) - lo[d]
+    Wi = np.take(W, np.nonzero(~ghost)[0], axis=ax)
+    assert np.array_equal(Wi, np.take(W0, np.nonzero(~ghost)[0], axis=ax))
+    G = np.take(W, np.nonzero(ghost)[0], axis=ax)
+    E = np.take(W0, [edge], axis=ax)
+    for c in range(13, 23):
+        assert np.array_equal(G[c], np.broadcast_to(E[c], G[c].shape)), c
