@@ -9,12 +9,9 @@
 // compared against both the reference's semantics and the fused HIP kernels.
 // Paths are relative to MHDAM-3.2.4/ of the reference.
 //
-// Parity unpinned (no closed form and no fixture of the reference checks
-// them; the device matches this file on them, so a mistake common to both
-// would pass): in helio_gs the ghost copies of neutral populations 2 and 3
-// (HELIOGS), and the same copies in the SW/LISM x pass; in swlism_reinit the
-// floor states of populations 2 and 3 (SWLISMREINIT).  Everything else is pinned by the
-// closed-form, invariant and reference-fixture tests beside it.
+// Pinned: every function on the path has a check of its own in tests/ --
+// closed forms, invariants, mutation-tested boundary and ghost-fill pins,
+// and the reference's own fixtures where it holds them.
 #include "mfm_oracle.h"
 
 #include <algorithm>
@@ -3127,6 +3124,15 @@ int oracle_fill_ghost(const mfm_params* p, mfm_fab Wf, int dir, double dx) {
   Eq E(p, 1.0, 1.0);
   Fab W; W.wrap(Wf);
   fill_ghost(E, W, dir, dx);
+  return MFM_OK;
+}
+// SWLISMREINIT on the conserved FAB U over box b, dx the cell size.
+int oracle_swlism_reinit(const mfm_params* p, mfm_fab Uf, mfm_box b, double dx) {
+  int rc = check_args(p);
+  if (rc) return rc;
+  Eq E(p, 1.0, 1.0);
+  Fab U; U.wrap(Uf);
+  swlism_reinit(E, U, dx, mkbox(b));
   return MFM_OK;
 }
 int oracle_fill_periodic(const mfm_params* p, mfm_fab Uf, mfm_box validb) {

@@ -82,6 +82,9 @@ void oracle_flux_vanleer(const mfm_params* p, const double* wl,
    primitive FAB (HELIOGS, LISMINIT and the axis reflection, the Riemann
    tube's copies); dx the cell size */
 int oracle_fill_ghost(const mfm_params* p, mfm_fab W, int dir, double dx);
+/* SWLISMREINIT (swlism/SWLISMF.ChF:305-403) on the conserved FAB U over
+   box b; dx the cell size */
+int oracle_swlism_reinit(const mfm_params* p, mfm_fab U, mfm_box b, double dx);
 /* BFLUX / BFLUXR (swlism/HeliosphericF.ChF:1363-1774) in face coordinates:
    win, wout = (rho, vn, vt1, vt2, p, Bn, Bt1, Bt2) inside and outside the
    boundary, dxyz the outward direction (BFLUXR only); out = 8 fluxes,

@@ -50,6 +50,7 @@ def load():
     lib.oracle_init_problem.argtypes = [P, mfmhd.Fab, mfmhd.Box, C.c_double]
     lib.oracle_fill_periodic.argtypes = [P, mfmhd.Fab, mfmhd.Box]
     lib.oracle_fill_ghost.argtypes = [P, mfmhd.Fab, C.c_int, C.c_double]
+    lib.oracle_swlism_reinit.argtypes = [P, mfmhd.Fab, mfmhd.Box, C.c_double]
     lib.oracle_flux_roe8.argtypes = [P, dp, dp, C.c_int, C.c_int, C.c_int, dp]
     lib.oracle_flux_rusanov.argtypes = [P, dp, dp, C.c_int, C.c_int, dp]
     lib.oracle_flux_dedner.argtypes = [P, C.c_double, dp, dp, C.c_int, C.c_int, dp]
@@ -171,6 +172,14 @@ def fill_ghost(p, W, lo, d, dx):
     rc = load().oracle_fill_ghost(C.byref(p), fab_of(W, lo), d, dx)
     assert rc == 0
     return W
+
+
+def swlism_reinit(p, U, lo, b_lo, b_hi, dx):
+    """SWLISMREINIT on the conserved FAB U (shape (nc, nz, ny, nx) at lo)
+    over the box [b_lo, b_hi], in place."""
+    rc = load().oracle_swlism_reinit(C.byref(p), fab_of(U, lo), mfmhd.Box.of(b_lo, b_hi), dx)
+    assert rc == 0
+    return U
 
 
 def _vec(x, n=32):

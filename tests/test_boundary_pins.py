@@ -377,3 +377,83 @@ def test_swlism_ghost_copies_of_populations_2_3(oracle_mod, side, d):
     E = np.take(W0, [edge], axis=ax)
     for c in range(13, 23):
         assert np.array_equal(G[c], np.broadcast_to(E[c], G[c].shape)), c
+
+
+# ---------------------------------------------------------- SWLISMREINIT --
+def test_swlism_reinit_states(oracle_mod):
+    """SWLISMREINIT (swlism/SWLISMF.ChF:305-403) inside R0: the plasma is the
+    radial supersonic wind (rho_sun / r^2, speed sun_v along r, pressure
+    p_sun / r^(2 gamma), no field), populations 2 and 3 are at rest at their
+    floor density and pressure (energy p / (gamma - 1)) and population 1 is
+    untouched; outside R0 nothing changes."""
+    nx, ny, dx = 48, 32, 1.0
+    p = mfmhd.swlism_params(sunXY=(0.4 * nx * dx, 0.0), R0=5.2 * dx, initR=8.0 * dx, dx=dx,
+                            num_cells=(nx, ny), nfluids=4)
+    nc = mfmhd.num_components(p)
+    rng = np.random.default_rng(130)
+    lo, hi = (10, 0, 0), (29, 9, 0)
+    U0 = rng.uniform(0.5, 2.0, (nc, 1, hi[1] + 1, hi[0] - lo[0] + 1))
+    U = oracle_mod.swlism_reinit(p, U0.copy(), lo, lo, hi, dx)
+    g = p.gamma
+    n_in = 0
+    for j in range(hi[1] + 1):
+        for i in range(lo[0], hi[0] + 1):
+            x, y = (i + 0.5) * dx - p.sun_xc, (j + 0.5) * dx - p.sun_yc
+            r = np.hypot(x, y)
+            u, u0 = U[:, 0, j, i - lo[0]], U0[:, 0, j, i - lo[0]]
+            if r > p.sun_r0:
+                assert np.array_equal(u, u0)
+                continue
+            n_in += 1
+            rho, pr = p.sun_rho / r ** 2, p.sun_p / r ** (2 * g)
+            want = [rho, rho * p.sun_v * x / r, rho * p.sun_v * y / r, 0.0,
+                    pr / (g - 1) + 0.5 * rho * p.sun_v ** 2, 0.0, 0.0, 0.0]
+            np.testing.assert_allclose(u[:8], want, rtol=1e-14, atol=0)
+            assert np.array_equal(u[8:13], u0[8:13])
+            for f, (dmin, pmin) in FLOORS.items():
+                b = 8 + 5 * (f - 1)
+                assert list(u[b:b + 5]) == [dmin, 0.0, 0.0, 0.0, pmin / (g - 1)], f
+    assert n_in > 20
+
+
+@pytest.mark.parametrize("side", [1, -1])
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_helio_bc_plasma(oracle_mod, d, side):
+    """HELIOBC's plasma flux (swlism/HeliosphericF.ChF:221-529): the high-x
+    face (the LISM inflow) carries the LISM state's physical flux and field;
+    the other faces take BFLUX of (interior, exterior) in the face frame, the
+    low faces as the point image of the high ones (velocity and field
+    reversed, mass and energy fluxes and the normal field negated back) and
+    the low-x face with BFLUXR's radial branch."""
+    p = helio4()
+    rng = np.random.default_rng(140 + 3 * d + side)
+    from test_oracle_pins import rand_prim
+    n, t1, t2 = 1 + d, 1 + (d + 1) % 3, 1 + (d + 2) % 3
+    for _ in range(8):
+        w_in, w_out = rand_prim(rng, nf=4), rand_prim(rng, nf=4)
+        wl, wr = (w_in, w_out) if side == 1 else (w_out, w_in)
+        ijk = [30, 33, 35]
+        ijk[d] = 63 if side == 1 else 0
+        f, bn = oracle_mod.helio_bc_face(p, side, d, 1.0 / 64, ijk, wl, wr)
+        if side == 1 and d == 0:
+            lism = np.array([1.0, p.lism_ux, p.lism_uy, p.lism_uz, p.lism_p,
+                             p.lism_bx, p.lism_by, p.lism_bz])
+            np.testing.assert_allclose(f[:8], mhd_flux(lism, G), rtol=1e-14, atol=1e-300)
+            assert bn == p.lism_bx
+            continue
+        sgn = 1.0 if side == 1 else -1.0
+
+        def frame(w):
+            return np.array([w[0], sgn * w[n], sgn * w[t1], sgn * w[t2], w[4],
+                             sgn * w[4 + n], sgn * w[4 + t1], sgn * w[4 + t2]])
+        radial = side == -1 and d == 0
+        dxyz = (0.0, 0.0, 0.0)
+        if radial:
+            dxyz = tuple(-((ijk[q] + 0.5) / 64 - (p.sun_xc, p.sun_yc, p.sun_zc)[q]) for q in range(3))
+        fl, bnv, _ = oracle_mod.helio_bflux(p, frame(w_in), frame(w_out), radial, dxyz)
+        want = np.zeros(8)
+        want[0], want[4] = sgn * fl[0], sgn * fl[4]
+        want[n], want[t1], want[t2] = fl[1], fl[2], fl[3]
+        want[4 + n], want[4 + t1], want[4 + t2] = fl[5], fl[6], fl[7]
+        np.testing.assert_allclose(f[:8], want, rtol=1e-14, atol=1e-300)
+        assert bn == sgn * bnv
