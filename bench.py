@@ -873,7 +873,7 @@ def main():
                     help="skip the level ghost-fill measurement")
     ap.add_argument("--stage", choices=["separate", "summed", "fused"], default=None,
                     help="kernel structure of the RK2 stages (default: the library's)")
-    ap.add_argument("--tz", type=int, default=16,
+    ap.add_argument("--tz", type=int, default=8,
                     help="planes per z-chunk of the tile kernels (summed, fused)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="after the run, launch the library's copy kernel over MIB MiB "

@@ -253,7 +253,7 @@ int mfm_ctx_get_timing(mfm_ctx* ctx, int stage, int kind, double* ms,
  * fluxes, then k_update.  MFM_STAGE_SUMMED: after k_prim and the
  * boundary-face kernel, one tile kernel per fluid group (the plasma with
  * psi and the normal B, then each neutral fluid) solves the interior faces
- * of a 64 x 8 column tile, z-chunks of tz planes (default 16), into LDS and
+ * of a 64 x 8 column tile, z-chunks of tz planes (default 8), into LDS and
  * writes per cell the face-flux differences summed over the directions;
  * k_update applies those sums (FP32-flux build only; the FP64 build keeps
  * the reference's order, the directions one at a time, and runs the

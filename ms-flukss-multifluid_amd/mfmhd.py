@@ -412,7 +412,7 @@ class Context:
     def set_walk(self, row_tile, xcd_order):
         self._check(self.lib.mfm_ctx_set_walk(self.h, int(row_tile), int(xcd_order)))
 
-    def set_stage_kernel(self, mode, tz=16):
+    def set_stage_kernel(self, mode, tz=8):
         """Kernel structure of RK2 / first-order stages: STAGE_SEPARATE,
         STAGE_FUSED (True), STAGE_SUMMED or STAGE_AUTO (the default: summed
         where the column tiles are well filled, FP32-flux build); tz =
