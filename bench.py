@@ -986,7 +986,12 @@ def main():
                    "cells": int(cells), "components": nc, "precision": args.precision,
                    "dt_control": "cfl %.2g, initial_cfl %.2g, max_dt_growth %.2f" % (args.cfl, CFL0, DT_GROWTH),
                    "walk": {"row_tile": walk[0], "xcd_order": walk[1]},
-                   "stage_kernels": structure},
+                   "stage_kernels": structure,
+                   "tile_kernels": ("%s column tiles, z-chunks of %s" %
+                                    ("64 x 8" if structure == "summed" else "16 x 16",
+                                     "%d planes" % args.tz if args.stage is not None
+                                     else "8 planes (library default)")
+                                    if structure in ("summed", "fused") else None)},
         "roofline": {"bound": "hbm", "scope": "whole RK2 step, %d B per cell-update" % step_bytes,
                      "achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": step_achieved / HBM_PEAK,
